@@ -1,0 +1,151 @@
+"""Throughput of the fused ideal-MHD step on one MI355X.
+
+Workload (BASELINE "plain method at scale"): the heliosphere set-up on a
+3-D 256^3 box [-1,1]^3 -- supersonic radial wind held inside r < 0.1, a
+uniform magnetised inflow (rho 0.1 with +-1 % seeded noise, u = -2.5,
+p = 0.1, |B| = 0.3 tilted 30 deg in x-y) elsewhere -- first-order HLLD,
+forward Euler, CFL 0.4, outflow boundaries, FP64.  Synthetic data.
+
+A step is one fused flux + update launch over all 16.8 M cells.  `value` is
+cell-updates/s over the timed steps (wall clock between device syncs, state
+resident in HBM).  The roofline object uses the kernel's own duration from
+HIP events on the library's stream and 128 algorithmic bytes per
+cell-update (read U^n, write U^{n+1}: 8 FP64 each).  The mixed-precision
+and Heun-RK2 forms are reported beside it under "forms".  cpu_baseline is
+the CPU oracle (the C restatement of the reference algorithm) on a bounded
+sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--n 256]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "HLLD cell-updates/s FP64 vs mixed (% HBM peak); p50/p99 solve latency"
+
+
+def traffic_from_profiles(name):
+    """HBM bytes per launch measured by rocprofv3 PMC runs, when a summary
+    for this kernel/workload is committed (profiles/<name>.txt holds a line
+    'traffic_bytes_per_launch: <int>')."""
+    p = os.path.join(ROOT, "profiles", name + ".txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":")[1]))
+    return None
+
+
+def run_form(g, W, precision, time_scheme, steps, warmup):
+    from set_amd import solver
+    cfg = solver.cartesian_config(cfl=0.4, r_hold=0.1, precision=precision,
+                                  time_scheme=time_scheme)
+    with solver.Context(cfg) as c:
+        c.upload_structured(g, W)
+        c.dt_from_state()
+        if warmup:
+            c.step(warmup)
+        c.sync()
+        t0 = time.perf_counter()
+        c.step(steps)
+        c.sync()
+        wall = time.perf_counter() - t0
+        ms_ev = c.step_timed(steps)  # kernel time on the library's stream
+        t_sim, dt = c.sync()
+    launches = steps * (2 if time_scheme == 1 else 1)
+    bytes_per_cell = 320 if time_scheme == 1 else 128
+    return {
+        "value": g.n_cell * steps / wall,
+        "ms_per_step": 1e3 * wall / steps,
+        "kernel_ms_per_launch": ms_ev / launches,
+        "achieved_GBs": g.n_cell * bytes_per_cell * steps / (ms_ev * 1e-3) / 1e9,
+        "t_sim": t_sim,
+        "dt": dt,
+    }
+
+
+def cpu_baseline(n, steps, threads):
+    import oracle
+    from set_amd import mesh
+    g = mesh.grid_3d(n, n, n)
+    fm = mesh.face_mesh(g)
+    W = mesh.heliosphere_state(g)
+    s = oracle.Solver(fm, W, oracle.cartesian_config(), nthreads=threads)
+    s.dt_next = s.dt_scan()
+    s.run(1)  # warm-up
+    t0 = time.perf_counter()
+    st = s.run(steps)
+    wall = time.perf_counter() - t0
+    assert st == 0
+    return {"value": g.n_cell * steps / wall, "unit": "cell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle (C restatement of chlld + face/cell passes) on the same "
+                      f"heliosphere set-up at {n}^3, {steps} forward-Euler steps, "
+                      f"{threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--forms", default="fp64,mixed,heun")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=128)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    from set_amd import mesh
+    n = args.n
+    g = mesh.grid_3d(n, n, n)
+    W = mesh.heliosphere_state(g)
+    forms = {}
+    wanted = args.forms.split(",")
+    fp = run_form(g, W, 0, 0, args.steps, args.warmup)
+    forms["fp64_euler"] = fp
+    if "mixed" in wanted:
+        forms["mixed_euler"] = run_form(g, W, 1, 0, args.steps, args.warmup)
+        forms["mixed_euler"]["speedup_vs_fp64"] = forms["mixed_euler"]["value"] / fp["value"]
+    if "heun" in wanted:
+        forms["fp64_heun"] = run_form(g, W, 0, 1, args.steps, args.warmup)
+    for f in forms.values():
+        f["hbm_frac"] = f["achieved_GBs"] / PEAK_HBM_GBS
+    del W
+
+    traffic = traffic_from_profiles(f"fused_fp64_euler_{n}cube")
+    achieved = fp["achieved_GBs"]
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM_GBS, "traffic": traffic}
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(args.cpu_n, args.cpu_steps, threads)
+    line = {
+        "metric": METRIC, "value": fp["value"], "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": fp["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded heliosphere initial state)",
+        "config": {"workload": f"heliosphere 3-D {n}^3 ideal MHD, first-order HLLD, forward "
+                               "Euler, CFL 0.4, box [-1,1]^3", "cells": g.n_cell,
+                   "bytes_per_cell_update": 128, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "forms": forms,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

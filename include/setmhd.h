@@ -1,0 +1,166 @@
+/*
+ * setmhd.h -- C ABI of the MI355X (gfx950) ideal-MHD finite-volume step.
+ *
+ * Drop-in for SET's CUDA-Fortran hot path (SergeyMSU/SET,
+ * SET_Fortran_Cuda_MHD/).  The reference's boundary is a set of module-global
+ * device arrays plus chevron launches:
+ *   Set_CUDA            cuf_kernel.cuf:351-382   -> setmhd_upload_mesh
+ *   time_step2 = 2e-9   cuf_kernel.cuf:758       -> setmhd_set_dt
+ *   step loop, 4 launches per step
+ *                       cuf_kernel.cuf:780-885   -> setmhd_step
+ *   Send_data_to_Host   cuf_kernel.cuf:384-400   -> setmhd_download
+ * The kernels CUF_MGD_grans(_RK2) / CUF_MGD_cell(s)(_RK2)
+ * (cuf_solvers.cuf:62-794) and chlld (Solvers.f90:1143-1859) run inside
+ * setmhd_step.  Every call returns an int status (SETMHD_OK = 0) where the
+ * reference printed an error string or executed STOP.
+ *
+ * Arrays cross the boundary in the reference's own layout
+ * (Storage_module.f90:16-37): Fortran column-major, component index
+ * fastest, 1-based cell/face numbers, -1 = no neighbour, 0 = no face slot.
+ * Cell_par holds primitives rho,u,v,w,p,Bx,By,Bz (B in SET's stored units,
+ * chlld divides by sqrt(4 pi)) followed by 8 neutral-H moments.
+ *
+ * One context owns its device memory and one HIP stream; contexts are
+ * independent and may be used from different host threads.
+ */
+#ifndef SETMHD_H
+#define SETMHD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum setmhd_status {
+  SETMHD_OK = 0,
+  SETMHD_ERR_ARG = 1,        /* bad argument / state                         */
+  SETMHD_ERR_HIP = 2,        /* a HIP runtime call failed                    */
+  SETMHD_ERR_NOMEM = 3,      /* device allocation failed                     */
+  SETMHD_ERR_RHO = 4,        /* density <= 0 after an update (ref: STOP,
+                                cuf_solvers.cuf:708-711)                    */
+  SETMHD_ERR_REGION = 5,     /* HLLD found no wave region (ref: STOP,
+                                Solvers.f90:1816-1825)                      */
+  SETMHD_ERR_FACE_RHO = 6,   /* negative face density after TVD (ref: STOP,
+                                cuf_solvers.cuf:414-420)                    */
+  SETMHD_ERR_UNSUPPORTED = 7 /* option not available on this path            */
+};
+
+/* flags (bitmask): the reference's heliosphere-specific semantics */
+#define SETMHD_F_ZERO_BW 1     /* w = B = 0 after update, hydro energy (cuf_solvers.cuf:725-733) */
+#define SETMHD_F_AXISYM 2      /* cylindrical y-sources /rr(2) (cuf_solvers.cuf:706-736)        */
+#define SETMHD_F_SOURCE_MK 4   /* charge-exchange source (cuf_kernel.cuf:215-288)               */
+#define SETMHD_F_HELIO_HACKS 8 /* LAX wedge :296-298, u<=-0.1 clamp :320-322, v clamp :783-785 */
+
+#define SETMHD_RIEMANN_LAX 0
+#define SETMHD_RIEMANN_HLL 1
+#define SETMHD_RIEMANN_HLLC 2
+#define SETMHD_RIEMANN_HLLD 3
+
+#define SETMHD_TIME_EULER 0 /* CUF_MGD_grans + CUF_MGD_cells (cuf_kernel.cuf:837-846) */
+#define SETMHD_TIME_HEUN 1  /* CUF_MGD_*_RK2 stages 1 and 2 (cuf_kernel.cuf:847-877) */
+
+#define SETMHD_PREC_FP64 0
+#define SETMHD_PREC_MIXED 1 /* FP32 face states / Riemann solver, FP64 state, p and dt */
+
+typedef struct setmhd_config {
+  int32_t riemann;     /* chlld n_state (cuf_solvers.cuf:260 uses 2 = HLLC) */
+  int32_t recon;       /* 0 first order, 1 minmod TVD (cuf_solvers.cuf:363-412) */
+  int32_t time_scheme; /* SETMHD_TIME_*                                      */
+  int32_t flags;       /* SETMHD_F_*                                         */
+  double cfl;          /* dt_face = cfl*dist/max(|SL|,|SR|); ref 0.2 (REAL*4) at :429 */
+  double dt_cap;       /* start value of the face min ('time = 100000.0', :259) */
+  double r_skip_face;  /* faces with |centre(s1)| < r skipped; ref 0.02 (:282-284) */
+  double r_hold_cell;  /* cells with |centre| < r held; ref 0.05 (:611)       */
+  int32_t precision;   /* SETMHD_PREC_* (structured path)                    */
+  int32_t device;      /* HIP device ordinal                                 */
+  int32_t tile_x;      /* structured tile width in cells (0 = default)       */
+  int32_t tile_y;      /* structured tile height (0 = default)               */
+} setmhd_config;
+
+typedef struct setmhd_ctx setmhd_ctx;
+
+/* Fills *cfg with the reference's production semantics (cuf_solvers.cuf:
+ * HLLC, minmod TVD, Heun RK2, axisymmetric + charge-exchange sources,
+ * w = B = 0, heliosphere clamps, CFL 0.2, skip radii 0.02/0.05). */
+void setmhd_config_reference(setmhd_config *cfg);
+
+/* Fills *cfg with plain ideal MHD on a Cartesian box: first-order HLLD,
+ * forward Euler, no sources, CFL number `cfl` on the cell width, cells
+ * inside r_hold held (the wind source). */
+void setmhd_config_cartesian(setmhd_config *cfg, double cfl, double r_hold);
+
+/* Creates a context on cfg->device.  Replaces the allocation half of
+ * Set_CUDA (cuf_kernel.cuf:351-382). */
+int setmhd_create(setmhd_ctx **out, const setmhd_config *cfg);
+void setmhd_destroy(setmhd_ctx *ctx);
+
+/* Unstructured drop-in: uploads SET's mesh and state exactly as Set_CUDA
+ * does (cuf_kernel.cuf:357-381; arrays of Storage_module.f90:16-37).
+ * gran_potok may be NULL (zeros).  Resets time to t0. */
+int setmhd_upload_mesh(setmhd_ctx *ctx, int32_t n_cell, int32_t n_gran,
+                       const double *cell_par /*[16*n_cell]*/,
+                       const double *cell_center /*[3*n_cell]*/,
+                       const double *cell_volume /*[n_cell]*/,
+                       const int32_t *cell_gran /*[6*n_cell]*/,
+                       const double *gran_normal /*[3*n_gran]*/,
+                       const double *gran_square /*[n_gran]*/,
+                       const double *gran_center /*[3*n_gran]*/,
+                       const int32_t *gran_neighbour /*[2*n_gran]*/,
+                       const int32_t *gran_neighbour_tvd /*[2*n_gran]*/,
+                       const int32_t *gran_type /*[n_gran]*/,
+                       const double *gran_potok /*[9*n_gran] or NULL*/,
+                       double t0);
+
+/* Structured Cartesian box nx*ny*nz (nz = 1: 2-D) on
+ * [box0,box1]x[box2,box3]x[box4,box5], outflow (copy) boundaries on every
+ * side.  cell_par is Cell_par for the cells numbered c = i + nx*(j+ny*k).
+ * The state is kept on the device as FP64 conserved variables, SoA. */
+int setmhd_upload_structured(setmhd_ctx *ctx, int32_t nx, int32_t ny,
+                             int32_t nz, const double *box /*[6]*/,
+                             const double *cell_par /*[16*nx*ny*nz]*/,
+                             double t0);
+
+/* time_step2 := dt (the dt the next step uses; reference 2e-9 at
+ * cuf_kernel.cuf:758). */
+int setmhd_set_dt(setmhd_ctx *ctx, double dt);
+
+/* CFL minimum over all faces of the current state (the face pass's
+ * reduction without an update); stores it as the next step's dt. */
+int setmhd_dt_from_state(setmhd_ctx *ctx, double *dt_out);
+
+/* Advances nsteps steps (cuf_kernel.cuf:780-885 without the dumps): dt of
+ * step n is the face minimum found in step n-1 (lagged, :789-790).  Runs
+ * asynchronously on the context's stream; the device error word is checked
+ * when the steps have finished. */
+int setmhd_step(setmhd_ctx *ctx, int32_t nsteps);
+
+/* As setmhd_step, bracketed by HIP events on the context's stream;
+ * *ms = elapsed device time of the nsteps steps. */
+int setmhd_step_timed(setmhd_ctx *ctx, int32_t nsteps, float *ms);
+
+/* Waits for the stream; returns the simulated time and the next dt. */
+int setmhd_sync(setmhd_ctx *ctx, double *t, double *dt_next);
+
+/* Send_data_to_Host (cuf_kernel.cuf:384-400): Cell_par (16*n_cell,
+ * variables 9..16 as uploaded) and the simulated time. */
+int setmhd_download(setmhd_ctx *ctx, double *cell_par, double *t);
+
+/* Gran_POTOK of the last face pass (9*n_gran, flux x area; drop-in only). */
+int setmhd_download_potok(setmhd_ctx *ctx, double *gran_potok);
+
+/* Number of cells / faces of the uploaded problem. */
+int setmhd_sizes(const setmhd_ctx *ctx, int64_t *n_cell, int64_t *n_gran);
+
+/* Relative density change of the last step, ||rho^{n+1}-rho^n||_1 /
+ * ||rho^n||_1 (structured path; serving convergence test). */
+int setmhd_residual(setmhd_ctx *ctx, double *res);
+
+/* Message of the last failure of this context ("" if none). */
+int setmhd_last_error(const setmhd_ctx *ctx, char *buf, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

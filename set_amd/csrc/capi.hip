@@ -1,0 +1,645 @@
+// capi.hip -- the C ABI of include/setmhd.h: context, device state, step
+// loop and error protocol.
+//
+// The reference keeps its state in module-global device arrays and drives
+// four synchronous launches per step from the host, with the dt scalar
+// round-tripped through the host every step (cuf_kernel.cuf:780-885).  Here
+// a context owns its arrays and one stream; a step is a sequence of
+// asynchronous launches whose dt and time live on the device (a three-slot
+// dt ring: step n reads slot n%3, min-reduces into slot (n+1)%3), so nothing
+// crosses to the host until setmhd_sync / setmhd_download.  The device
+// error word replaces the reference's in-kernel STOP.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/setmhd.h"
+#include "kernels.hpp"
+
+using namespace setmhd;
+
+struct setmhd_ctx {
+  setmhd_config cfg;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string msg;
+  int kind = 0;  // 0 none, 1 unstructured, 2 structured
+  long long n_cell = 0, n_gran = 0;
+  long long step = 0;  // steps done since upload (selects the dt slot)
+  // device scalars: dt ring [3], time [1], error word, residual [2]
+  double* d_scal = nullptr;
+  int* d_err = nullptr;
+  // unstructured
+  UMesh um{};
+  double *d_W = nullptr, *d_W2 = nullptr, *d_H = nullptr, *d_potok = nullptr;
+  void* d_mesh = nullptr;
+  // structured
+  SGrid sg{};
+  double *d_U = nullptr, *d_U2 = nullptr;  // Euler: ping-pong; Heun: U^n, U^1
+  int cur_buf = 0;                         // Euler: which buffer holds U^n
+  double* d_H16 = nullptr;                 // Cell_par(9:16) kept for download
+  int kchunk = 32;
+};
+
+namespace {
+
+int fail(setmhd_ctx* c, int code, const std::string& m) {
+  if (c) c->msg = m;
+  return code;
+}
+
+#define HIPCHK(c, call)                                                                   \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail((c), SETMHD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+int dalloc(setmhd_ctx* c, T** p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) return fail(c, SETMHD_ERR_NOMEM, "hipMalloc failed");
+  return SETMHD_OK;
+}
+
+void free_state(setmhd_ctx* c) {
+  for (double** p : {&c->d_W, &c->d_W2, &c->d_H, &c->d_potok, &c->d_U, &c->d_U2, &c->d_H16}) {
+    if (*p) hipFree(*p);
+    *p = nullptr;
+  }
+  if (c->d_mesh) hipFree(c->d_mesh);
+  c->d_mesh = nullptr;
+  c->kind = 0;
+}
+
+DtRing ring(setmhd_ctx* c) { return DtRing{c->d_scal, c->d_scal + 3}; }
+
+// Reads and clears the device error word; maps it to a status.
+int check_err(setmhd_ctx* c) {
+  int h = 0;
+  HIPCHK(c, hipMemcpyAsync(&h, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (h == 0) return SETMHD_OK;
+  HIPCHK(c, hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+  if (h & kErrRho) return fail(c, SETMHD_ERR_RHO, "density <= 0 after update");
+  if (h & kErrRegion) return fail(c, SETMHD_ERR_REGION, "HLLD: no wave region (non-finite state)");
+  return fail(c, SETMHD_ERR_FACE_RHO, "negative face density after reconstruction");
+}
+
+__global__ void k_prim_to_cons(const double* __restrict__ cp, long long nc, double* __restrict__ U) {
+#pragma clang fp contract(off)
+  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const double* w = cp + 16 * c;
+  const double spi4 = sqrt(kCpi4);
+  double rho = w[0], u = w[1], v = w[2], ww = w[3], p = w[4];
+  double bx = w[5] / spi4, by = w[6] / spi4, bz = w[7] / spi4;
+  U[c] = rho;
+  U[nc + c] = rho * u;
+  U[2 * nc + c] = rho * v;
+  U[3 * nc + c] = rho * ww;
+  U[4 * nc + c] = p / (kGamma - 1.0) + 0.5 * rho * ((u * u + v * v) + ww * ww) +
+                  0.5 * ((bx * bx + by * by) + bz * bz);
+  U[5 * nc + c] = bx;
+  U[6 * nc + c] = by;
+  U[7 * nc + c] = bz;
+}
+
+__global__ void k_cons_to_prim(const double* __restrict__ U, long long nc, double* __restrict__ cp) {
+#pragma clang fp contract(off)
+  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const double spi4 = sqrt(kCpi4);
+  double rho = U[c];
+  double u = U[nc + c] / rho, v = U[2 * nc + c] / rho, w = U[3 * nc + c] / rho;
+  double bx = U[5 * nc + c], by = U[6 * nc + c], bz = U[7 * nc + c];
+  double p = (kGamma - 1.0) * (U[4 * nc + c] - 0.5 * rho * ((u * u + v * v) + w * w) -
+                               0.5 * ((bx * bx + by * by) + bz * bz));
+  double* o = cp + 16 * c;
+  o[0] = rho;
+  o[1] = u;
+  o[2] = v;
+  o[3] = w;
+  o[4] = p;
+  o[5] = bx * spi4;
+  o[6] = by * spi4;
+  o[7] = bz * spi4;
+}
+
+// AoS16 <-> AoS8 column split for the unstructured state
+__global__ void k_split16(const double* __restrict__ cp, long long n, double* __restrict__ W,
+                          double* __restrict__ H) {
+  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    W[8 * c + k] = cp[16 * c + k];
+    H[8 * c + k] = cp[16 * c + 8 + k];
+  }
+}
+__global__ void k_join16(const double* __restrict__ W, const double* __restrict__ H, long long n,
+                         double* __restrict__ cp) {
+  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    cp[16 * c + k] = W[8 * c + k];
+    cp[16 * c + 8 + k] = H[8 * c + k];
+  }
+}
+
+// structured: keep Cell_par(9:16) for download
+__global__ void k_take_h(const double* __restrict__ cp, long long n, double* __restrict__ H) {
+  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+#pragma unroll
+  for (int k = 0; k < 8; k++) H[8 * c + k] = cp[16 * c + 8 + k];
+}
+__global__ void k_put_h(const double* __restrict__ H, long long n, double* __restrict__ cp) {
+  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+#pragma unroll
+  for (int k = 0; k < 8; k++) cp[16 * c + 8 + k] = H[8 * c + k];
+}
+
+unsigned nblocks(long long n) { return (unsigned)((n + 255) / 256); }
+
+// dt ring re-armed for "next step uses dt": slot (step%3) = dt,
+// slot ((step+1)%3) = 1e15.
+int arm_dt(setmhd_ctx* c, double dt) {
+  double h[3];
+  int cur = (int)(c->step % 3);
+  HIPCHK(c, hipMemcpyAsync(h, c->d_scal, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  h[cur] = dt;
+  h[(cur + 1) % 3] = kDtReset;
+  HIPCHK(c, hipMemcpyAsync(c->d_scal, h, 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SETMHD_OK;
+}
+
+int upload_scalars(setmhd_ctx* c, double t0) {
+  double h[6] = {2.0e-9, kDtReset, kDtReset, t0, 0.0, 0.0};  // cuf_kernel.cuf:758
+  c->step = 0;
+  HIPCHK(c, hipMemcpyAsync(c->d_scal, h, sizeof h, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SETMHD_OK;
+}
+
+UCfg ucfg(const setmhd_config& k) {
+  return UCfg{k.riemann, k.recon, k.flags, k.cfl, k.dt_cap, k.r_skip_face, k.r_hold_cell};
+}
+
+int tile_x(const setmhd_ctx* c) { return c->cfg.tile_x > 0 ? c->cfg.tile_x : 32; }
+int tile_y(const setmhd_ctx* c) { return c->cfg.tile_y > 0 ? c->cfg.tile_y : 8; }
+
+// one structured step; dt-only scan is mode -1 (no write)
+int structured_step(setmhd_ctx* c, double* resid) {
+  FusedArgs a{};
+  a.g = c->sg;
+  a.dt = ring(c);
+  a.cur = (int)(c->step % 3);
+  a.kchunk = c->kchunk;
+  a.resid = resid;
+  a.err = c->d_err;
+  const int prec = c->cfg.precision;
+  hipError_t e;
+  if (c->cfg.time_scheme == SETMHD_TIME_EULER) {
+    double* in = c->cur_buf == 0 ? c->d_U : c->d_U2;
+    double* out = c->cur_buf == 0 ? c->d_U2 : c->d_U;
+    a.Uf = in;
+    a.Un = in;
+    a.Uo = out;
+    e = launch_fused(c->stream, a, kModeEuler, prec, tile_x(c), tile_y(c));
+    if (e != hipSuccess) return fail(c, SETMHD_ERR_HIP, std::string("fused: ") + hipGetErrorString(e));
+    c->cur_buf ^= 1;
+  } else {
+    a.Uf = c->d_U;
+    a.Un = c->d_U;
+    a.Uo = c->d_U2;
+    a.resid = nullptr;
+    e = launch_fused(c->stream, a, kModeHeun1, prec, tile_x(c), tile_y(c));
+    if (e != hipSuccess) return fail(c, SETMHD_ERR_HIP, std::string("fused: ") + hipGetErrorString(e));
+    a.Uf = c->d_U2;
+    a.Un = c->d_U;
+    a.Uo = c->d_U;
+    a.resid = resid;
+    e = launch_fused(c->stream, a, kModeHeun2, prec, tile_x(c), tile_y(c));
+    if (e != hipSuccess) return fail(c, SETMHD_ERR_HIP, std::string("fused: ") + hipGetErrorString(e));
+  }
+  c->step++;
+  return SETMHD_OK;
+}
+
+int unstructured_step(setmhd_ctx* c) {
+  const UCfg k = ucfg(c->cfg);
+  const int cur = (int)(c->step % 3);
+  DtRing r = ring(c);
+  hipError_t e;
+  if (c->cfg.time_scheme == SETMHD_TIME_EULER) {  // cuf_kernel.cuf:837-846
+    e = launch_face_pass(c->stream, c->um, k, c->d_W, c->d_potok, r, cur, 1, 1, c->d_err);
+    if (e == hipSuccess)
+      e = launch_cell_pass(c->stream, c->um, k, 0, c->d_W, c->d_W2, c->d_H, c->d_potok, r, cur,
+                           c->d_err);
+  } else {  // :847-877
+    e = launch_face_pass(c->stream, c->um, k, c->d_W, c->d_potok, r, cur, 1, 1, c->d_err);
+    if (e == hipSuccess)
+      e = launch_cell_pass(c->stream, c->um, k, 1, c->d_W, c->d_W2, c->d_H, c->d_potok, r, cur,
+                           c->d_err);
+    if (e == hipSuccess)
+      e = launch_face_pass(c->stream, c->um, k, c->d_W2, c->d_potok, r, cur, 0, 0, c->d_err);
+    if (e == hipSuccess)
+      e = launch_cell_pass(c->stream, c->um, k, 2, c->d_W, c->d_W2, c->d_H, c->d_potok, r, cur,
+                           c->d_err);
+  }
+  if (e != hipSuccess) return fail(c, SETMHD_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
+  c->step++;
+  return SETMHD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void setmhd_config_reference(setmhd_config* k) {
+  std::memset(k, 0, sizeof *k);
+  k->riemann = SETMHD_RIEMANN_HLLC;  // cuf_solvers.cuf:260
+  k->recon = 1;                      // :363-412
+  k->time_scheme = SETMHD_TIME_HEUN; // cuf_kernel.cuf:847-877
+  k->flags = SETMHD_F_ZERO_BW | SETMHD_F_AXISYM | SETMHD_F_SOURCE_MK | SETMHD_F_HELIO_HACKS;
+  k->cfl = SETMHD_F4(0.2);           // cuf_solvers.cuf:429
+  k->dt_cap = 1.0e5;                 // :259
+  k->r_skip_face = SETMHD_F4(0.02);  // :282
+  k->r_hold_cell = SETMHD_F4(0.05);  // :611
+  k->precision = SETMHD_PREC_FP64;
+}
+
+void setmhd_config_cartesian(setmhd_config* k, double cfl, double r_hold) {
+  std::memset(k, 0, sizeof *k);
+  k->riemann = SETMHD_RIEMANN_HLLD;
+  k->recon = 0;
+  k->time_scheme = SETMHD_TIME_EULER;
+  k->flags = 0;
+  k->cfl = 2.0 * cfl;  // CFL number on the cell width h = 2 * dist
+  k->dt_cap = 1.0e5;
+  k->r_skip_face = -1.0;
+  k->r_hold_cell = r_hold;
+  k->precision = SETMHD_PREC_FP64;
+}
+
+int setmhd_create(setmhd_ctx** out, const setmhd_config* cfg) {
+  if (!out || !cfg) return SETMHD_ERR_ARG;
+  *out = nullptr;
+  if (cfg->riemann < 0 || cfg->riemann > 3 || cfg->recon < 0 || cfg->recon > 1 ||
+      cfg->time_scheme < 0 || cfg->time_scheme > 1 || cfg->precision < 0 || cfg->precision > 1)
+    return SETMHD_ERR_ARG;
+  setmhd_ctx* c = new setmhd_ctx();
+  c->cfg = *cfg;
+  c->device = cfg->device;
+  auto bad = [&](int code) {
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return code;
+  };
+  if (hipSetDevice(c->device) != hipSuccess) return bad(SETMHD_ERR_HIP);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bad(SETMHD_ERR_HIP);
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+    return bad(SETMHD_ERR_HIP);
+  if (hipMalloc((void**)&c->d_scal, 8 * sizeof(double)) != hipSuccess) return bad(SETMHD_ERR_NOMEM);
+  if (hipMalloc((void**)&c->d_err, sizeof(int)) != hipSuccess) return bad(SETMHD_ERR_NOMEM);
+  if (upload_scalars(c, 0.0) != SETMHD_OK) return bad(SETMHD_ERR_HIP);
+  *out = c;
+  return SETMHD_OK;
+}
+
+void setmhd_destroy(setmhd_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  free_state(c);
+  if (c->d_scal) hipFree(c->d_scal);
+  if (c->d_err) hipFree(c->d_err);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int setmhd_upload_mesh(setmhd_ctx* c, int32_t n_cell, int32_t n_gran, const double* cell_par,
+                       const double* cell_center, const double* cell_volume,
+                       const int32_t* cell_gran, const double* gran_normal,
+                       const double* gran_square, const double* gran_center,
+                       const int32_t* gran_neighbour, const int32_t* gran_neighbour_tvd,
+                       const int32_t* gran_type, const double* gran_potok, double t0) {
+  if (!c) return SETMHD_ERR_ARG;
+  if (n_cell <= 0 || n_gran <= 0 || !cell_par || !cell_center || !cell_volume || !cell_gran ||
+      !gran_normal || !gran_square || !gran_center || !gran_neighbour || !gran_neighbour_tvd ||
+      !gran_type)
+    return fail(c, SETMHD_ERR_ARG, "upload_mesh: null array or empty mesh");
+  if (c->cfg.precision != SETMHD_PREC_FP64)
+    return fail(c, SETMHD_ERR_UNSUPPORTED, "the face-list path is FP64 only");
+  // The validation of Set_Storage (Storage_module.f90:92-156): neighbour 1
+  // in 1..N_cell, neighbour 2 and TVD neighbours <= N_cell, density >= 1e-9.
+  for (int32_t f = 0; f < n_gran; f++) {
+    int32_t s1 = gran_neighbour[2 * f], s2 = gran_neighbour[2 * f + 1];
+    if (s1 < 1 || s1 > n_cell || s2 > n_cell || s2 == 0 || s2 < -1 ||
+        gran_neighbour_tvd[2 * f] > n_cell || gran_neighbour_tvd[2 * f + 1] > n_cell)
+      return fail(c, SETMHD_ERR_ARG, "upload_mesh: face neighbour out of range");
+  }
+  for (int32_t i = 0; i < n_cell; i++) {
+    if (cell_par[16 * (size_t)i] < 0.000000001)
+      return fail(c, SETMHD_ERR_ARG, "upload_mesh: rho < 1e-9");
+    for (int s = 0; s < 6; s++)
+      if (cell_gran[6 * (size_t)i + s] < 0 || cell_gran[6 * (size_t)i + s] > n_gran)
+        return fail(c, SETMHD_ERR_ARG, "upload_mesh: cell face number out of range");
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  free_state(c);
+  const size_t nc = n_cell, nf = n_gran;
+  // one block for the geometry
+  size_t bytes = nc * (3 + 1) * 8 + nf * (3 + 1 + 3) * 8 + nc * 6 * 4 + nf * (2 + 2 + 1) * 4 + 64;
+  char* base = nullptr;
+  if (hipMalloc((void**)&base, bytes) != hipSuccess) return fail(c, SETMHD_ERR_NOMEM, "hipMalloc mesh");
+  c->d_mesh = base;
+  size_t off = 0;
+  auto take = [&](size_t n) {
+    char* p = base + off;
+    off += (n + 7) & ~size_t(7);
+    return p;
+  };
+  double* cc = (double*)take(nc * 3 * 8);
+  double* cv = (double*)take(nc * 8);
+  double* gn = (double*)take(nf * 3 * 8);
+  double* gs = (double*)take(nf * 8);
+  double* gc = (double*)take(nf * 3 * 8);
+  int* cg = (int*)take(nc * 6 * 4);
+  int* gnb = (int*)take(nf * 2 * 4);
+  int* gtv = (int*)take(nf * 2 * 4);
+  int* gty = (int*)take(nf * 4);
+  HIPCHK(c, hipMemcpyAsync(cc, cell_center, nc * 3 * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(cv, cell_volume, nc * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(gn, gran_normal, nf * 3 * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(gs, gran_square, nf * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(gc, gran_center, nf * 3 * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(cg, cell_gran, nc * 6 * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(gnb, gran_neighbour, nf * 2 * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(gtv, gran_neighbour_tvd, nf * 2 * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(gty, gran_type, nf * 4, hipMemcpyHostToDevice, c->stream));
+  c->um = UMesh{n_cell, n_gran, cc, cv, cg, gn, gs, gc, gnb, gtv, gty};
+  int st;
+  if ((st = dalloc(c, &c->d_W, nc * 8)) || (st = dalloc(c, &c->d_W2, nc * 8)) ||
+      (st = dalloc(c, &c->d_H, nc * 8)) || (st = dalloc(c, &c->d_potok, nf * 8)))
+    return st;
+  double* stage = nullptr;
+  if ((st = dalloc(c, &stage, nc * 16))) return st;
+  HIPCHK(c, hipMemcpyAsync(stage, cell_par, nc * 16 * 8, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_split16, dim3(nblocks(nc)), dim3(256), 0, c->stream, stage, (long long)nc,
+                     c->d_W, c->d_H);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(c->d_W2, c->d_W, nc * 8 * 8, hipMemcpyDeviceToDevice, c->stream));  // :361
+  if (gran_potok) {
+    std::vector<double> p8(nf * 8);
+    for (size_t f = 0; f < nf; f++)
+      for (int k = 0; k < 8; k++) p8[8 * f + k] = gran_potok[9 * f + k];
+    HIPCHK(c, hipMemcpyAsync(c->d_potok, p8.data(), nf * 8 * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  } else {
+    HIPCHK(c, hipMemsetAsync(c->d_potok, 0, nf * 8 * 8, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  hipFree(stage);
+  c->n_cell = n_cell;
+  c->n_gran = n_gran;
+  c->kind = 1;
+  return upload_scalars(c, t0);
+}
+
+int setmhd_upload_structured(setmhd_ctx* c, int32_t nx, int32_t ny, int32_t nz, const double* box,
+                             const double* cell_par, double t0) {
+  if (!c) return SETMHD_ERR_ARG;
+  if (nx < 1 || ny < 1 || nz < 1 || !box || !cell_par)
+    return fail(c, SETMHD_ERR_ARG, "upload_structured: bad size or null array");
+  if (c->cfg.riemann != SETMHD_RIEMANN_HLLD || c->cfg.recon != 0 ||
+      (c->cfg.flags & (SETMHD_F_ZERO_BW | SETMHD_F_AXISYM | SETMHD_F_SOURCE_MK | SETMHD_F_HELIO_HACKS)))
+    return fail(c, SETMHD_ERR_UNSUPPORTED,
+                "structured path: first-order HLLD ideal MHD without heliosphere switches");
+  const long long nc = (long long)nx * ny * nz;
+  if (nc > (1LL << 31) - 1) return fail(c, SETMHD_ERR_ARG, "upload_structured: grid too large");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  free_state(c);
+  int st;
+  if ((st = dalloc(c, &c->d_U, (size_t)nc * 8)) || (st = dalloc(c, &c->d_U2, (size_t)nc * 8)) ||
+      (st = dalloc(c, &c->d_H16, (size_t)nc * 8)))
+    return st;
+  // stage Cell_par through the second state buffer's memory when it is big
+  // enough (8*nc doubles < 16*nc): use a temporary instead.
+  double* stage = nullptr;
+  if ((st = dalloc(c, &stage, (size_t)nc * 16))) return st;
+  HIPCHK(c, hipMemcpyAsync(stage, cell_par, (size_t)nc * 16 * 8, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_prim_to_cons, dim3(nblocks(nc)), dim3(256), 0, c->stream, stage, nc, c->d_U);
+  HIPCHK(c, hipGetLastError());
+  hipLaunchKernelGGL(k_take_h, dim3(nblocks(nc)), dim3(256), 0, c->stream, stage, nc, c->d_H16);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(c->d_U2, c->d_U, (size_t)nc * 8 * 8, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  hipFree(stage);
+  SGrid& g = c->sg;
+  g = SGrid{};
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  g.nc = nc;
+  g.x0 = box[0];
+  g.y0 = box[2];
+  g.z0 = box[4];
+  g.dx = (box[1] - box[0]) / nx;
+  g.dy = (box[3] - box[2]) / ny;
+  g.dz = nz > 1 ? (box[5] - box[4]) / nz : 1.0;
+  if (nz > 1) {
+    g.ax = g.dy * g.dz;
+    g.ay = g.dx * g.dz;
+    g.az = g.dx * g.dy;
+    g.inv_vol = 1.0 / (g.dx * g.dy * g.dz);
+  } else {
+    g.z0 = 0.0;
+    g.ax = g.dy;
+    g.ay = g.dx;
+    g.az = 0.0;
+    g.inv_vol = 1.0 / (g.dx * g.dy);
+  }
+  g.distx = 0.5 * g.dx;
+  g.disty = 0.5 * g.dy;
+  g.distz = 0.5 * g.dz;
+  g.cfl = c->cfg.cfl;
+  g.dt_cap = c->cfg.dt_cap;
+  g.r_hold = c->cfg.r_hold_cell;
+  c->cur_buf = 0;
+  c->n_cell = nc;
+  c->n_gran = nz > 1 ? (long long)(nx + 1) * ny * nz + (long long)nx * (ny + 1) * nz +
+                           (long long)nx * ny * (nz + 1)
+                     : (long long)(nx + 1) * ny + (long long)nx * (ny + 1);
+  c->kind = 2;
+  return upload_scalars(c, t0);
+}
+
+int setmhd_set_dt(setmhd_ctx* c, double dt) {
+  if (!c || c->kind == 0) return SETMHD_ERR_ARG;
+  if (!(dt > 0.0)) return fail(c, SETMHD_ERR_ARG, "set_dt: dt must be > 0");
+  HIPCHK(c, hipSetDevice(c->device));
+  return arm_dt(c, dt);
+}
+
+int setmhd_dt_from_state(setmhd_ctx* c, double* dt_out) {
+  if (!c || c->kind == 0) return SETMHD_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  // Run the face reduction on the current state without an update: the
+  // unstructured face pass writes only fluxes; for the structured path an
+  // Euler pass into the spare buffer (its output is discarded).
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double save[6];
+  HIPCHK(c, hipMemcpy(save, c->d_scal, sizeof save, hipMemcpyDeviceToHost));
+  const int cur = (int)(c->step % 3);
+  double h[6] = {save[0], save[1], save[2], save[3], save[4], save[5]};
+  h[(cur + 1) % 3] = kDtReset;
+  HIPCHK(c, hipMemcpy(c->d_scal, h, sizeof h, hipMemcpyHostToDevice));
+  hipError_t e;
+  if (c->kind == 1) {
+    e = launch_face_pass(c->stream, c->um, ucfg(c->cfg), c->d_W, c->d_potok, ring(c), cur, 1, 0,
+                         c->d_err);
+  } else {
+    FusedArgs a{};
+    a.g = c->sg;
+    a.dt = ring(c);
+    a.cur = cur;
+    a.kchunk = c->kchunk;
+    a.err = c->d_err;
+    double* in = c->cur_buf == 0 ? c->d_U : c->d_U2;
+    double* spare = c->cur_buf == 0 ? c->d_U2 : c->d_U;
+    if (c->cfg.time_scheme == SETMHD_TIME_HEUN) {
+      in = c->d_U;
+      spare = c->d_U2;
+    }
+    a.Uf = in;
+    a.Un = in;
+    a.Uo = spare;
+    e = launch_fused(c->stream, a, kModeHeun1, c->cfg.precision, tile_x(c), tile_y(c));
+  }
+  if (e != hipSuccess) return fail(c, SETMHD_ERR_HIP, hipGetErrorString(e));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double r[6];
+  HIPCHK(c, hipMemcpy(r, c->d_scal, sizeof r, hipMemcpyDeviceToHost));
+  double dt = r[(cur + 1) % 3];
+  // restore the ring and time, then arm dt for the next step
+  save[cur] = dt;
+  save[(cur + 1) % 3] = kDtReset;
+  HIPCHK(c, hipMemcpy(c->d_scal, save, sizeof save, hipMemcpyHostToDevice));
+  if (dt_out) *dt_out = dt;
+  return check_err(c);
+}
+
+int setmhd_step(setmhd_ctx* c, int32_t nsteps) {
+  if (!c || c->kind == 0 || nsteps < 0) return SETMHD_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  for (int32_t s = 0; s < nsteps; s++) {
+    int st = c->kind == 1 ? unstructured_step(c) : structured_step(c, nullptr);
+    if (st) return st;
+  }
+  return check_err(c);
+}
+
+int setmhd_step_timed(setmhd_ctx* c, int32_t nsteps, float* ms) {
+  if (!c || c->kind == 0 || nsteps < 0) return SETMHD_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  for (int32_t s = 0; s < nsteps; s++) {
+    int st = c->kind == 1 ? unstructured_step(c) : structured_step(c, nullptr);
+    if (st) return st;
+  }
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  if (ms) HIPCHK(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
+  return check_err(c);
+}
+
+int setmhd_residual(setmhd_ctx* c, double* res) {
+  if (!c || c->kind != 2 || !res) return SETMHD_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  double* r = c->d_scal + 4;
+  HIPCHK(c, hipMemsetAsync(r, 0, 2 * sizeof(double), c->stream));
+  int st = structured_step(c, r);
+  if (st) return st;
+  double h[2];
+  HIPCHK(c, hipMemcpyAsync(h, r, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *res = h[1] > 0.0 ? h[0] / h[1] : 0.0;
+  return check_err(c);
+}
+
+int setmhd_sync(setmhd_ctx* c, double* t, double* dt_next) {
+  if (!c) return SETMHD_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  double h[4];
+  HIPCHK(c, hipMemcpyAsync(h, c->d_scal, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (t) *t = h[3];
+  if (dt_next) *dt_next = h[c->step % 3];
+  return check_err(c);
+}
+
+int setmhd_download(setmhd_ctx* c, double* cell_par, double* t) {
+  if (!c || c->kind == 0 || !cell_par) return SETMHD_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const long long nc = c->n_cell;
+  double* stage = nullptr;
+  int st;
+  if ((st = dalloc(c, &stage, (size_t)nc * 16))) return st;
+  if (c->kind == 1) {
+    hipLaunchKernelGGL(k_join16, dim3(nblocks(nc)), dim3(256), 0, c->stream, c->d_W, c->d_H, nc, stage);
+  } else {
+    const double* U = (c->cfg.time_scheme == SETMHD_TIME_EULER && c->cur_buf == 1) ? c->d_U2 : c->d_U;
+    hipLaunchKernelGGL(k_cons_to_prim, dim3(nblocks(nc)), dim3(256), 0, c->stream, U, nc, stage);
+    hipLaunchKernelGGL(k_put_h, dim3(nblocks(nc)), dim3(256), 0, c->stream, c->d_H16, nc, stage);
+  }
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(cell_par, stage, (size_t)nc * 16 * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(stage);
+  if (e != hipSuccess) return fail(c, SETMHD_ERR_HIP, std::string("download: ") + hipGetErrorString(e));
+  return setmhd_sync(c, t, nullptr);
+}
+
+int setmhd_download_potok(setmhd_ctx* c, double* gran_potok) {
+  if (!c || c->kind != 1 || !gran_potok) return SETMHD_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<double> p8((size_t)c->n_gran * 8);
+  HIPCHK(c, hipMemcpyAsync(p8.data(), c->d_potok, p8.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (long long f = 0; f < c->n_gran; f++) {
+    for (int k = 0; k < 8; k++) gran_potok[9 * f + k] = p8[8 * f + k];
+    gran_potok[9 * f + 8] = 0.0;  // cuf_solvers.cuf:433
+  }
+  return SETMHD_OK;
+}
+
+int setmhd_sizes(const setmhd_ctx* c, int64_t* n_cell, int64_t* n_gran) {
+  if (!c) return SETMHD_ERR_ARG;
+  if (n_cell) *n_cell = c->n_cell;
+  if (n_gran) *n_gran = c->n_gran;
+  return SETMHD_OK;
+}
+
+int setmhd_last_error(const setmhd_ctx* c, char* buf, size_t n) {
+  if (!c || !buf || n == 0) return SETMHD_ERR_ARG;
+  std::snprintf(buf, n, "%s", c->msg.c_str());
+  return SETMHD_OK;
+}
+
+}  // extern "C"

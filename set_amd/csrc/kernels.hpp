@@ -1,0 +1,65 @@
+// kernels.hpp -- device-side data descriptors and launchers shared by the
+// C ABI (capi.hip) and the kernel translation units.
+#pragma once
+#include "common.hpp"
+
+namespace setmhd {
+
+// SET's face-list mesh on the device (Storage_module.f90:16-37), 1-based.
+struct UMesh {
+  int n_cell, n_gran;
+  const double* cell_center;  // [3*Nc]
+  const double* cell_volume;  // [Nc]
+  const int* cell_gran;       // [6*Nc]
+  const double* gran_normal;  // [3*Nf]
+  const double* gran_square;  // [Nf]
+  const double* gran_center;  // [3*Nf]
+  const int* gran_nb;         // [2*Nf]
+  const int* gran_tvd;        // [2*Nf]
+  const int* gran_type;       // [Nf]
+};
+
+struct UCfg {
+  int riemann, recon, flags;
+  double cfl, dt_cap, r_skip, r_hold;
+};
+
+hipError_t launch_face_pass(hipStream_t s, const UMesh& m, const UCfg& c, const double* S,
+                            double* potok, DtRing dt, int cur, int reduce, int book, int* err);
+hipError_t launch_cell_pass(hipStream_t s, const UMesh& m, const UCfg& c, int stage, double* W,
+                            double* W2, const double* H, const double* potok, DtRing dt, int cur,
+                            int* err);
+
+// A uniform Cartesian box: state is FP64 conserved, one field of nc
+// doubles per variable (rho, mx, my, mz, E, bx, by, bz), x fastest.
+struct SGrid {
+  int nx, ny, nz;       // nz == 1 for 2-D
+  long long nc;         // nx*ny*nz, field stride
+  double x0, y0, z0;    // box origin
+  double dx, dy, dz;    // spacing (dz = 1 in 2-D)
+  double ax, ay, az;    // face areas (2-D: ax = dy, ay = dx)
+  double inv_vol;       // 1 / cell volume
+  double distx, disty, distz;  // face-to-centre distance (CFL length)
+  double cfl, dt_cap, r_hold;
+};
+
+enum : int { kModeEuler = 0, kModeHeun1 = 1, kModeHeun2 = 2 };
+
+struct FusedArgs {
+  SGrid g;
+  const double* Uf;  // state whose face fluxes are taken
+  const double* Un;  // U^n (Heun stage 2)
+  double* Uo;        // output state
+  DtRing dt;
+  int cur;           // dt slot of this step
+  int kchunk;        // z-planes per block (3-D)
+  double* resid;     // [2]: sum |drho|, sum |rho| (or null)
+  int* err;
+};
+
+// precision: 0 FP64, 1 mixed.  Returns hipErrorInvalidValue for an
+// unsupported tile shape.
+hipError_t launch_fused(hipStream_t s, const FusedArgs& a, int mode, int precision, int tile_x,
+                        int tile_y);
+
+}  // namespace setmhd

@@ -1,0 +1,428 @@
+// structured.hip -- fused flux + conservative update on a uniform Cartesian
+// box, for gfx950.
+//
+// Semantics are those of SET's face pass + cell pass (cuf_solvers.cuf:62-794)
+// with first-order states, an HLLD face flux and the lagged CFL minimum of
+// cuf_kernel.cuf:789-790, restated on a structured grid:
+//   * one launch per Euler step or Heun stage: face fluxes are formed once in
+//     LDS and applied to both neighbour cells in the same kernel, so the only
+//     HBM traffic is the state itself (read once, written once);
+//   * 2.5-D blocking: a block owns a TX x TY column of cells and marches up a
+//     chunk of z-planes.  Each plane (+1-cell halo ring) is staged in LDS as
+//     face-ready data (rho, u, b, total pressure, energy, fast speeds); the
+//     x- and y-faces of the plane are read from LDS, the z-face below each
+//     cell comes from the previous plane kept in registers;
+//   * face work is balanced: every thread forms one x-face and one y-face of
+//     the plane and one z-face of its column; the TX + TY faces on the far
+//     tile edges go to the first TX + TY lanes;
+//   * the CFL candidate is the per-axis max wave speed (min of c/s over faces
+//     = c/max s, division being monotone), reduced per wave by shuffles and
+//     per block through LDS, then one atomic per block;
+//   * blocks are remapped so each XCD (blockIdx % 8) gets a contiguous range
+//     of tiles: neighbouring tiles share halo lines in that XCD's L2.
+// Outflow (copy) boundaries come from clamping neighbour indices.  Cells
+// with |centre| < r_hold are held (the wind source), as the reference holds
+// |centre| < 0.05 (cuf_solvers.cuf:611).  Precision: R = double, or R = float
+// for the face states and Riemann solver with the state, the pressure
+// recovery, the flux accumulation and the dt reduction kept in FP64.
+#include "hlld_fast.hpp"
+#include "kernels.hpp"
+
+namespace setmhd {
+
+// face-data slots in LDS
+enum { QR = 0, QU = 1, QB = 4, QPT = 7, QE = 8, QCF = 9, NQ = 11 };
+
+template <class R>
+struct CellQ {
+  R rho, u[3], b[3], pt, e, cf[3];
+};
+
+// conserved FP64 state -> face data.  Pressure is recovered in FP64:
+// p = (gamma-1)(E - rho|u|^2/2 - |b|^2/2).
+template <class R, bool NEED_CFZ>
+__device__ __forceinline__ void derive(const double U[8], CellQ<R>& q) {
+  const double ir = frcp(U[0]);
+  const double u = U[1] * ir, v = U[2] * ir, w = U[3] * ir;
+  const double bb = U[5] * U[5] + U[6] * U[6] + U[7] * U[7];
+  const double ke2 = U[1] * u + U[2] * v + U[3] * w;
+  const double p = (kGamma - 1.0) * (U[4] - 0.5 * ke2 - 0.5 * bb);
+  q.rho = R(U[0]);
+  q.u[0] = R(u);
+  q.u[1] = R(v);
+  q.u[2] = R(w);
+  q.b[0] = R(U[5]);
+  q.b[1] = R(U[6]);
+  q.b[2] = R(U[7]);
+  q.pt = R(p + 0.5 * bb);
+  q.e = R(U[4]);
+  const R irr = R(ir);
+  const R c2 = R(kGamma) * R(p) * irr;
+  const R s = c2 + R(bb) * irr;
+  q.cf[0] = cf_cell<R>(c2, s, q.b[0] * q.b[0] * irr);
+  q.cf[1] = cf_cell<R>(c2, s, q.b[1] * q.b[1] * irr);
+  if (NEED_CFZ) q.cf[2] = cf_cell<R>(c2, s, q.b[2] * q.b[2] * irr);
+}
+
+template <class R>
+__device__ __forceinline__ Side<R> side_of(const CellQ<R>& q, int d) {
+  const int d1 = d == 2 ? 0 : d + 1, d2 = d == 0 ? 2 : d - 1;
+  Side<R> s;
+  s.rho = q.rho;
+  s.vn = q.u[d];
+  s.vt1 = q.u[d1];
+  s.vt2 = q.u[d2];
+  s.bn = q.b[d];
+  s.bt1 = q.b[d1];
+  s.bt2 = q.b[d2];
+  s.pt = q.pt;
+  s.e = q.e;
+  s.cf = q.cf[d];
+  return s;
+}
+
+template <class R, int TX, int TY>
+struct Smem {
+  R q[NQ][TY + 2][TX + 2];
+  R fx[8][TY][TX + 1];
+  R fy[8][TY + 1][TX];
+};
+
+template <class R, int TX, int TY>
+__device__ __forceinline__ void put_q(Smem<R, TX, TY>& sm, int ly, int lx, const CellQ<R>& q) {
+  sm.q[QR][ly][lx] = q.rho;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    sm.q[QU + c][ly][lx] = q.u[c];
+    sm.q[QB + c][ly][lx] = q.b[c];
+  }
+  sm.q[QPT][ly][lx] = q.pt;
+  sm.q[QE][ly][lx] = q.e;
+  sm.q[QCF][ly][lx] = q.cf[0];
+  sm.q[QCF + 1][ly][lx] = q.cf[1];
+}
+
+// face-frame side of the LDS cell (ly, lx) for an x- (d = 0) or y- (d = 1) face
+template <class R, int TX, int TY>
+__device__ __forceinline__ Side<R> get_side(const Smem<R, TX, TY>& sm, int ly, int lx, int d) {
+  const int d1 = d + 1, d2 = d == 0 ? 2 : 0;
+  Side<R> s;
+  s.rho = sm.q[QR][ly][lx];
+  s.vn = sm.q[QU + d][ly][lx];
+  s.vt1 = sm.q[QU + d1][ly][lx];
+  s.vt2 = sm.q[QU + d2][ly][lx];
+  s.bn = sm.q[QB + d][ly][lx];
+  s.bt1 = sm.q[QB + d1][ly][lx];
+  s.bt2 = sm.q[QB + d2][ly][lx];
+  s.pt = sm.q[QPT][ly][lx];
+  s.e = sm.q[QE][ly][lx];
+  s.cf = sm.q[QCF + d][ly][lx];
+  return s;
+}
+
+__device__ __forceinline__ bool held_cell(const SGrid& g, double xc, double yc, double zc) {
+  // |centre| < r_hold with the centre and norm rounded as the host forms
+  // them (no contraction): same held set as the face-list mesh.
+  if (fabs(xc) >= g.r_hold || fabs(yc) >= g.r_hold || fabs(zc) >= g.r_hold) return false;
+  double r2 = __dadd_rn(__dadd_rn(__dmul_rn(xc, xc), __dmul_rn(yc, yc)), __dmul_rn(zc, zc));
+  return __dsqrt_rn(r2) < g.r_hold;
+}
+
+template <int MODE>
+__device__ __forceinline__ void finalize(const FusedArgs& a, double dt, long long idx, bool held,
+                                         const double Uold[8], const double acc[8],
+                                         double& rs_d, double& rs_r) {
+  const SGrid& g = a.g;
+  if (held) {
+    if (MODE != kModeHeun2) {
+#pragma unroll
+      for (int v = 0; v < 8; v++) a.Uo[v * g.nc + idx] = Uold[v];
+    }
+    return;
+  }
+  const double f = dt * g.inv_vol;
+  double Un[8];
+  if (MODE == kModeHeun2) {
+#pragma unroll
+    for (int v = 0; v < 8; v++) {
+      double un = a.Un[v * g.nc + idx];
+      Un[v] = 0.5 * (un + Uold[v] - f * acc[v]);
+      if (v == 0) {
+        rs_d += fabs(Un[0] - un);
+        rs_r += fabs(un);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < 8; v++) Un[v] = Uold[v] - f * acc[v];
+    rs_d += fabs(Un[0] - Uold[0]);
+    rs_r += fabs(Uold[0]);
+  }
+  if (!(Un[0] > 0.0)) atomicOr(a.err, kErrRho);  // cuf_solvers.cuf:708-711
+  // p <= 0 -> 1e-7 (cuf_solvers.cuf:777-779), applied to the total energy
+  const double ir = frcp(Un[0]);
+  const double ke = 0.5 * (Un[1] * Un[1] + Un[2] * Un[2] + Un[3] * Un[3]) * ir;
+  const double me = 0.5 * (Un[5] * Un[5] + Un[6] * Un[6] + Un[7] * Un[7]);
+  if ((kGamma - 1.0) * (Un[4] - ke - me) <= 0.0)
+    Un[4] = SETMHD_F4(0.0000001) / (kGamma - 1.0) + ke + me;
+#pragma unroll
+  for (int v = 0; v < 8; v++) a.Uo[v * g.nc + idx] = Un[v];
+}
+
+template <class R, int TX, int TY, int DIM, int MODE>
+__global__ __launch_bounds__(TX* TY) void k_fused(FusedArgs a) {
+  constexpr int NH = 2 * TX + 2 * TY;
+  __shared__ Smem<R, TX, TY> sm;
+  __shared__ double red[TX * TY / 64];
+  const SGrid& g = a.g;
+  const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+
+  // tile of this block; XCD-contiguous remap when the grid divides by 8
+  const int ntx = (g.nx + TX - 1) / TX, nty = (g.ny + TY - 1) / TY;
+  int b = blockIdx.x;
+  const int nblk = gridDim.x;
+  if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
+  const int tile_x = b % ntx, rest = b / ntx, tile_y = rest % nty, kc = rest / nty;
+  const int i0 = tile_x * TX, j0 = tile_y * TY;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool own = i < g.nx && j < g.ny;
+  const int ic = min(i, g.nx - 1), jc = min(j, g.ny - 1);
+  const int k0 = DIM == 3 ? kc * a.kchunk : 0;
+  const int k1 = DIM == 3 ? min(k0 + a.kchunk, g.nz) : 1;
+  const long long plane = (long long)g.nx * g.ny;
+  const double dt = a.dt.slot[a.cur];
+  if (MODE != kModeHeun2 && blockIdx.x == 0 && tid == 0) {
+    a.dt.t[0] = a.dt.t[0] + dt;                 // cuf_kernel.cuf:792-794
+    a.dt.slot[(a.cur + 2) % 3] = kDtReset;      // :790, for step n+2
+  }
+  const double xc = __dadd_rn(g.x0, __dmul_rn((double)ic + 0.5, g.dx));
+  const double yc = __dadd_rn(g.y0, __dmul_rn((double)jc + 0.5, g.dy));
+
+  // halo cell of this thread (ring around the tile, corners excluded)
+  int hly = 0, hlx = 0, hgi = 0, hgj = 0;
+  const bool has_h = tid < NH;
+  if (has_h) {
+    if (tid < TX) { hlx = tid + 1; hly = 0; hgi = i0 + tid; hgj = j0 - 1; }
+    else if (tid < 2 * TX) { hlx = tid - TX + 1; hly = TY + 1; hgi = i0 + tid - TX; hgj = j0 + TY; }
+    else if (tid < 2 * TX + TY) { hlx = 0; hly = tid - 2 * TX + 1; hgi = i0 - 1; hgj = j0 + tid - 2 * TX; }
+    else { hlx = TX + 1; hly = tid - 2 * TX - TY + 1; hgi = i0 + TX; hgj = j0 + tid - 2 * TX - TY; }
+    hgi = min(max(hgi, 0), g.nx - 1);
+    hgj = min(max(hgj, 0), g.ny - 1);
+  }
+
+  double smax[3] = {0.0, 0.0, 0.0};
+  double rs_d = 0.0, rs_r = 0.0;
+  double Up[8], accp[8];
+  Side<R> zp;
+  bool ok = true;
+
+  if (DIM == 3) {  // z-face data of the plane below the chunk
+    const int kb = max(k0 - 1, 0);
+    const long long idx = ic + (long long)g.nx * jc + plane * kb;
+    double U[8];
+#pragma unroll
+    for (int v = 0; v < 8; v++) U[v] = a.Uf[v * g.nc + idx];
+    CellQ<R> q;
+    derive<R, true>(U, q);
+    zp = side_of(q, 2);
+  }
+
+  for (int k = k0; k < k1; ++k) {
+    const long long idx = ic + (long long)g.nx * jc + plane * k;
+    double Uc[8];
+#pragma unroll
+    for (int v = 0; v < 8; v++) Uc[v] = a.Uf[v * g.nc + idx];
+    CellQ<R> qc;
+    derive<R, DIM == 3>(Uc, qc);
+    put_q(sm, ty + 1, tx + 1, qc);
+    if (has_h) {
+      const long long hidx = hgi + (long long)g.nx * hgj + plane * k;
+      double Uh[8];
+#pragma unroll
+      for (int v = 0; v < 8; v++) Uh[v] = a.Uf[v * g.nc + hidx];
+      CellQ<R> qh;
+      derive<R, false>(Uh, qh);
+      put_q(sm, hly, hlx, qh);
+    }
+    __syncthreads();
+
+    // x- and y-faces of the plane (+ far-edge faces on the first lanes)
+    const int ntask = tid < TX + TY ? 3 : 2;
+    for (int t = 0; t < ntask; t++) {
+      int d, ly, lx, ry, rx;
+      if (t == 0) { d = 0; ly = ty + 1; lx = tx; ry = ty + 1; rx = tx + 1; }
+      else if (t == 1) { d = 1; ly = ty; lx = tx + 1; ry = ty + 1; rx = tx + 1; }
+      else if (tid < TY) { d = 0; ly = tid + 1; lx = TX; ry = tid + 1; rx = TX + 1; }
+      else { d = 1; ly = TY; lx = tid - TY + 1; ry = TY + 1; rx = tid - TY + 1; }
+      const Side<R> L = get_side(sm, ly, lx, d);
+      const Side<R> Rr = get_side(sm, ry, rx, d);
+      R F[8], s;
+      ok &= hlld(L, Rr, F, s);
+      smax[d] = fmax(smax[d], (double)s);
+      const int d1 = d + 1, d2 = d == 0 ? 2 : 0;
+      // face-frame components -> x,y,z
+      R Fx[8];
+      Fx[0] = F[0];
+      Fx[1 + d] = F[1];
+      Fx[1 + d1] = F[2];
+      Fx[1 + d2] = F[3];
+      Fx[4] = F[4];
+      Fx[5 + d] = F[5];
+      Fx[5 + d1] = F[6];
+      Fx[5 + d2] = F[7];
+      if (d == 0) {
+        const int fy = ry - 1, fxi = rx - 1;  // face index: left of LDS col rx
+#pragma unroll
+        for (int v = 0; v < 8; v++) sm.fx[v][fy][fxi] = Fx[v];
+      } else {
+        const int fy = ry - 1, fxi = rx - 1;  // face below LDS row ry
+#pragma unroll
+        for (int v = 0; v < 8; v++) sm.fy[v][fy][fxi] = Fx[v];
+      }
+    }
+
+    R Fz[8];
+    Side<R> zc;
+    if (DIM == 3) {  // z-face below this cell
+      zc = side_of(qc, 2);
+      R s;
+      ok &= hlld(zp, zc, Fz, s);
+      smax[2] = fmax(smax[2], (double)s);
+    }
+    __syncthreads();
+
+    // signed face sum in the face-slot order (x-, x+, y-, y+, z-, z+)
+    double acc[8];
+#pragma unroll
+    for (int v = 0; v < 8; v++) {
+      double s = -((double)sm.fx[v][ty][tx] * g.ax);
+      s = s + (double)sm.fx[v][ty][tx + 1] * g.ax;
+      s = s - (double)sm.fy[v][ty][tx] * g.ay;
+      s = s + (double)sm.fy[v][ty + 1][tx] * g.ay;
+      acc[v] = s;
+    }
+    if (DIM == 3) {
+      // z-face: rotate frame (z, x, y) back to x, y, z
+      double fz[8];
+      fz[0] = Fz[0];
+      fz[3] = Fz[1];
+      fz[1] = Fz[2];
+      fz[2] = Fz[3];
+      fz[4] = Fz[4];
+      fz[7] = Fz[5];
+      fz[5] = Fz[6];
+      fz[6] = Fz[7];
+#pragma unroll
+      for (int v = 0; v < 8; v++) acc[v] = acc[v] - fz[v] * g.az;
+      if (k > k0 && own) {
+#pragma unroll
+        for (int v = 0; v < 8; v++) accp[v] = accp[v] + fz[v] * g.az;
+        const double zcen = __dadd_rn(g.z0, __dmul_rn((double)(k - 1) + 0.5, g.dz));
+        finalize<MODE>(a, dt, idx - plane, held_cell(g, xc, yc, zcen), Up, accp, rs_d, rs_r);
+      }
+#pragma unroll
+      for (int v = 0; v < 8; v++) {
+        Up[v] = Uc[v];
+        accp[v] = acc[v];
+      }
+      zp = zc;
+    } else if (own) {
+      finalize<MODE>(a, dt, idx, held_cell(g, xc, yc, 0.0), Uc, acc, rs_d, rs_r);
+    }
+  }
+
+  if (DIM == 3) {  // z-face above the chunk's last plane
+    const int ke = min(k1, g.nz - 1);
+    const long long idx = ic + (long long)g.nx * jc + plane * ke;
+    double U[8];
+#pragma unroll
+    for (int v = 0; v < 8; v++) U[v] = a.Uf[v * g.nc + idx];
+    CellQ<R> q;
+    derive<R, true>(U, q);
+    const Side<R> zc = side_of(q, 2);
+    R Fz[8], s;
+    ok &= hlld(zp, zc, Fz, s);
+    smax[2] = fmax(smax[2], (double)s);
+    if (own) {
+      double fz[8];
+      fz[0] = Fz[0];
+      fz[3] = Fz[1];
+      fz[1] = Fz[2];
+      fz[2] = Fz[3];
+      fz[4] = Fz[4];
+      fz[7] = Fz[5];
+      fz[5] = Fz[6];
+      fz[6] = Fz[7];
+#pragma unroll
+      for (int v = 0; v < 8; v++) accp[v] = accp[v] + fz[v] * g.az;
+      const double zcen = __dadd_rn(g.z0, __dmul_rn((double)(k1 - 1) + 0.5, g.dz));
+      finalize<MODE>(a, dt, ic + (long long)g.nx * jc + plane * (k1 - 1),
+                     held_cell(g, xc, yc, zcen), Up, accp, rs_d, rs_r);
+    }
+  }
+  if (!ok) atomicOr(a.err, kErrRegion);
+
+  const int lane = tid & 63, wid = tid >> 6;
+  if (MODE != kModeHeun2) {  // CFL minimum for the next step
+    double cand = g.cfl * g.distx / smax[0];
+    cand = fmin(cand, g.cfl * g.disty / smax[1]);
+    if (DIM == 3) cand = fmin(cand, g.cfl * g.distz / smax[2]);
+    cand = fmin(cand, g.dt_cap);
+    cand = wave_min(cand);
+    if (lane == 0) red[wid] = cand;
+    __syncthreads();
+    if (tid == 0) {
+      double m = red[0];
+      for (int w = 1; w < TX * TY / 64; w++) m = fmin(m, red[w]);
+      atomic_min_pos(&a.dt.slot[(a.cur + 1) % 3], m);
+    }
+  }
+  if (a.resid != nullptr && MODE != kModeHeun1) {
+    rs_d = wave_sum(rs_d);
+    rs_r = wave_sum(rs_r);
+    if (lane == 0) {
+      atomicAdd(&a.resid[0], rs_d);
+      atomicAdd(&a.resid[1], rs_r);
+    }
+  }
+}
+
+template <class R, int TX, int TY, int DIM, int MODE>
+static hipError_t launch_one(hipStream_t s, const FusedArgs& a) {
+  const int ntx = (a.g.nx + TX - 1) / TX, nty = (a.g.ny + TY - 1) / TY;
+  const int nkz = DIM == 3 ? (a.g.nz + a.kchunk - 1) / a.kchunk : 1;
+  const long long nb = (long long)ntx * nty * nkz;
+  hipLaunchKernelGGL((k_fused<R, TX, TY, DIM, MODE>), dim3((unsigned)nb), dim3(TX * TY), 0, s, a);
+  return hipGetLastError();
+}
+
+template <class R, int TX, int TY>
+static hipError_t launch_tile(hipStream_t s, const FusedArgs& a, int mode) {
+  const bool d3 = a.g.nz > 1;
+  switch (mode) {
+    case kModeEuler:
+      return d3 ? launch_one<R, TX, TY, 3, kModeEuler>(s, a) : launch_one<R, TX, TY, 2, kModeEuler>(s, a);
+    case kModeHeun1:
+      return d3 ? launch_one<R, TX, TY, 3, kModeHeun1>(s, a) : launch_one<R, TX, TY, 2, kModeHeun1>(s, a);
+    default:
+      return d3 ? launch_one<R, TX, TY, 3, kModeHeun2>(s, a) : launch_one<R, TX, TY, 2, kModeHeun2>(s, a);
+  }
+}
+
+template <class R>
+static hipError_t launch_prec(hipStream_t s, const FusedArgs& a, int mode, int tx, int ty) {
+  if (tx == 32 && ty == 8) return launch_tile<R, 32, 8>(s, a, mode);
+  if (tx == 16 && ty == 16) return launch_tile<R, 16, 16>(s, a, mode);
+  if (tx == 64 && ty == 4) return launch_tile<R, 64, 4>(s, a, mode);
+  if (tx == 16 && ty == 8) return launch_tile<R, 16, 8>(s, a, mode);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_fused(hipStream_t s, const FusedArgs& a, int mode, int precision, int tile_x,
+                        int tile_y) {
+  if (a.g.nz > 1 && a.kchunk < 1) return hipErrorInvalidValue;
+  return precision == 0 ? launch_prec<double>(s, a, mode, tile_x, tile_y)
+                        : launch_prec<float>(s, a, mode, tile_x, tile_y);
+}
+
+}  // namespace setmhd
