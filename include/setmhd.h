@@ -77,6 +77,7 @@ typedef struct setmhd_config {
   int32_t device;      /* HIP device ordinal                                 */
   int32_t tile_x;      /* structured tile width in cells (0 = default)       */
   int32_t tile_y;      /* structured tile height (0 = default)               */
+  int32_t kchunk;      /* structured 3-D: z-planes per block (0 = default 32) */
 } setmhd_config;
 
 typedef struct setmhd_ctx setmhd_ctx;
@@ -87,7 +88,9 @@ typedef struct setmhd_ctx setmhd_ctx;
 void setmhd_config_reference(setmhd_config *cfg);
 
 /* Fills *cfg with plain ideal MHD on a Cartesian box: first-order HLLD,
- * forward Euler, no sources, CFL number `cfl` on the cell width, cells
+ * forward Euler, no sources, the reference's dt formula
+ * cfl*dist/max(|SL|,|SR|) (cuf_solvers.cuf:429) with its 0.2 replaced by
+ * `cfl` (dist = half the cell width, so 0.4 gives dt = 0.2 h/max|S|), cells
  * inside r_hold held (the wind source). */
 void setmhd_config_cartesian(setmhd_config *cfg, double cfl, double r_hold);
 

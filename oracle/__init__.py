@@ -45,10 +45,10 @@ def reference_config():
 
 def cartesian_config(riemann=3, recon=0, time_scheme=0, cfl=0.4, r_hold=0.1):
     """Ideal MHD on a Cartesian box: first-order HLLD, forward Euler, no
-    sources, cells inside r_hold held.  CFL number C on the face-normal cell
-    width h = 2*dist: cfl coefficient 2*C on dist."""
+    sources, cells inside r_hold held.  dt = cfl*dist/max(|SL|,|SR|), the
+    reference's formula (cuf_solvers.cuf:429) with its 0.2 replaced by cfl."""
     return Config(riemann=riemann, recon=recon, time_scheme=time_scheme, flags=0,
-                  cfl=2.0 * cfl, dt_cap=1.0e5, r_skip_face=-1.0, r_hold_cell=r_hold)
+                  cfl=cfl, dt_cap=1.0e5, r_skip_face=-1.0, r_hold_cell=r_hold)
 
 
 _lib = None

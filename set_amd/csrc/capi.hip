@@ -286,7 +286,9 @@ void setmhd_config_cartesian(setmhd_config* k, double cfl, double r_hold) {
   k->recon = 0;
   k->time_scheme = SETMHD_TIME_EULER;
   k->flags = 0;
-  k->cfl = 2.0 * cfl;  // CFL number on the cell width h = 2 * dist
+  // the reference's dt formula cfl*dist/max(|SL|,|SR|) (cuf_solvers.cuf:429)
+  // with its constant 0.2 replaced by `cfl`; dist = h/2 on a uniform grid
+  k->cfl = cfl;
   k->dt_cap = 1.0e5;
   k->r_skip_face = -1.0;
   k->r_hold_cell = r_hold;
@@ -296,12 +298,13 @@ void setmhd_config_cartesian(setmhd_config* k, double cfl, double r_hold) {
 int setmhd_create(setmhd_ctx** out, const setmhd_config* cfg) {
   if (!out || !cfg) return SETMHD_ERR_ARG;
   *out = nullptr;
-  if (cfg->riemann < 0 || cfg->riemann > 3 || cfg->recon < 0 || cfg->recon > 1 ||
+  if (cfg->riemann < 0 || cfg->riemann > 3 || cfg->recon < 0 || cfg->recon > 1 || cfg->kchunk < 0 ||
       cfg->time_scheme < 0 || cfg->time_scheme > 1 || cfg->precision < 0 || cfg->precision > 1)
     return SETMHD_ERR_ARG;
   setmhd_ctx* c = new setmhd_ctx();
   c->cfg = *cfg;
   c->device = cfg->device;
+  if (cfg->kchunk > 0) c->kchunk = cfg->kchunk;
   auto bad = [&](int code) {
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;

@@ -1,0 +1,133 @@
+! fcmhd_driver.f90 -- the solver's Fortran entry point on MI355X.
+!
+! Same control flow as SET's program MIK (FCMHD.f90:121-250) with
+! CUDA_START_MGD (cuf_kernel.cuf:402-949): read the FCMHD input file
+! (Set_Storage, Storage_module.f90:43-160), optionally the restart file
+! (Fill_data, :191-209), move mesh and state to the device (Set_CUDA),
+! run the step loop with the lagged dt starting from 2e-9, print progress
+! every print_step steps, bring the state back (Send_data_to_Host) and write
+! the output file (Save_Storage, :163-189).  Every kernel launch goes
+! through the C ABI (module setmhd).
+!
+! usage: fcmhd_driver input.bin output.bin all_step [print_step] [restart.bin] [cartesian]
+program fcmhd_driver
+  use iso_c_binding
+  use setmhd
+  implicit none
+
+  real(8) :: host_time_all, vv, t, dt_next
+  integer(4) :: host_N_cell, host_N_gran
+  real(8), allocatable :: host_Cell_par(:, :), host_Cell_center(:, :), host_Cell_Volume(:)
+  integer(4), allocatable :: host_Cell_gran(:, :)
+  real(8), allocatable :: host_Gran_normal(:, :), host_Gran_square(:), host_Gran_center(:, :)
+  integer(4), allocatable :: host_Gran_neighbour(:, :), host_Gran_neighbour_TVD(:, :)
+  integer(4), allocatable :: host_Gran_type(:)
+  real(8), allocatable :: host_Gran_POTOK(:, :)
+  character(len=512) :: fin, fout, arg, frestart
+  integer :: all_step, print_step, step, chunk, nargs, u
+  logical :: cartesian
+  type(setmhd_config) :: cfg
+  type(c_ptr) :: ctx
+  integer(c_int) :: st
+
+  nargs = command_argument_count()
+  if (nargs < 3) then
+    print *, 'usage: fcmhd_driver input.bin output.bin all_step [print_step] [restart.bin] [cartesian]'
+    stop 2
+  end if
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  call get_command_argument(3, arg)
+  read(arg, *) all_step
+  print_step = 50000
+  if (nargs >= 4) then
+    call get_command_argument(4, arg)
+    read(arg, *) print_step
+  end if
+  frestart = ''
+  if (nargs >= 5) call get_command_argument(5, frestart)
+  cartesian = .false.
+  if (nargs >= 6) then
+    call get_command_argument(6, arg)
+    cartesian = trim(arg) == 'cartesian'
+  end if
+
+  ! ---- Set_Storage --------------------------------------------------------
+  open(newunit=u, file=trim(fin), form='unformatted', access='stream', action='read', status='old')
+  read(u) host_time_all
+  read(u) host_N_cell
+  read(u) host_N_gran
+  allocate(host_Cell_par(16, host_N_cell), host_Cell_center(3, host_N_cell))
+  allocate(host_Cell_Volume(host_N_cell), host_Cell_gran(6, host_N_cell))
+  allocate(host_Gran_normal(3, host_N_gran), host_Gran_square(host_N_gran))
+  allocate(host_Gran_center(3, host_N_gran), host_Gran_neighbour(2, host_N_gran))
+  allocate(host_Gran_neighbour_TVD(2, host_N_gran), host_Gran_type(host_N_gran))
+  allocate(host_Gran_POTOK(9, host_N_gran))
+  read(u) host_Cell_par
+  read(u) host_Cell_center
+  read(u) host_Cell_Volume
+  read(u) host_Cell_gran
+  read(u) host_Gran_normal
+  read(u) host_Gran_square
+  read(u) host_Gran_center
+  read(u) host_Gran_neighbour
+  read(u) host_Gran_neighbour_TVD
+  read(u) vv
+  if (vv /= 121.0_8) stop 'FCMHD input: sentinel 121 missing'
+  read(u) host_Gran_type
+  read(u) vv
+  if (vv /= 122.0_8) stop 'FCMHD input: sentinel 122 missing'
+  read(u) host_Gran_POTOK
+  read(u) vv
+  if (vv /= 123.0_8) stop 'FCMHD input: sentinel 123 missing'
+  close(u)
+  print *, 'N_cell = ', host_N_cell, ' N_gran = ', host_N_gran
+
+  ! ---- Fill_data (restart) ------------------------------------------------
+  if (len_trim(frestart) > 0) then
+    open(newunit=u, file=trim(frestart), form='unformatted', access='stream', action='read', status='old')
+    read(u) host_time_all
+    read(u) host_Cell_par
+    read(u) vv
+    close(u)
+    if (vv /= 321.0_8) stop 'restart file: sentinel 321 missing'
+  end if
+
+  ! ---- Set_CUDA -------------------------------------------------------------
+  if (cartesian) then
+    call setmhd_config_cartesian(cfg, 0.4_8, 0.1_8)
+  else
+    call setmhd_config_reference(cfg)
+  end if
+  st = setmhd_create(ctx, cfg)
+  if (st /= SETMHD_OK) stop 'setmhd_create failed'
+  st = setmhd_upload_mesh(ctx, host_N_cell, host_N_gran, host_Cell_par, host_Cell_center, &
+       host_Cell_Volume, host_Cell_gran, host_Gran_normal, host_Gran_square, host_Gran_center, &
+       host_Gran_neighbour, host_Gran_neighbour_TVD, host_Gran_type, host_Gran_POTOK, host_time_all)
+  call setmhd_check(ctx, st, 'upload_mesh')
+  st = setmhd_set_dt(ctx, 0.000000002_8)   ! time_step2, cuf_kernel.cuf:758
+  call setmhd_check(ctx, st, 'set_dt')
+
+  ! ---- CUDA_START_MGD step loop ----------------------------------------------
+  step = 0
+  do while (step < all_step)
+    chunk = min(print_step, all_step - step)
+    st = setmhd_step(ctx, int(chunk, c_int32_t))
+    call setmhd_check(ctx, st, 'step')
+    step = step + chunk
+    st = setmhd_sync(ctx, t, dt_next)
+    call setmhd_check(ctx, st, 'sync')
+    print *, 'Step = ', step, '  step_time = ', dt_next, ' all time = ', t
+  end do
+
+  ! ---- Send_data_to_Host + Save_Storage --------------------------------------
+  st = setmhd_download(ctx, host_Cell_par, host_time_all)
+  call setmhd_check(ctx, st, 'download')
+  call setmhd_destroy(ctx)
+  open(newunit=u, file=trim(fout), form='unformatted', access='stream', action='write', status='replace')
+  write(u) host_time_all
+  write(u) host_Cell_par
+  write(u) 321.0_8
+  close(u)
+  print *, 'saved ', trim(fout), ' time = ', host_time_all
+end program fcmhd_driver

@@ -51,7 +51,8 @@ class Config(ctypes.Structure):
     """setmhd_config (include/setmhd.h)."""
     _fields_ = [("riemann", I32), ("recon", I32), ("time_scheme", I32), ("flags", I32),
                 ("cfl", D), ("dt_cap", D), ("r_skip_face", D), ("r_hold_cell", D),
-                ("precision", I32), ("device", I32), ("tile_x", I32), ("tile_y", I32)]
+                ("precision", I32), ("device", I32), ("tile_x", I32), ("tile_y", I32),
+                ("kchunk", I32)]
 
 
 _lib = None
@@ -103,7 +104,7 @@ def reference_config():
 
 
 def cartesian_config(cfl=0.4, r_hold=0.1, precision=PREC_FP64, time_scheme=TIME_EULER,
-                     tile=(0, 0), device=0):
+                     tile=(0, 0), device=0, kchunk=0):
     """First-order HLLD ideal MHD on a box (setmhd_config_cartesian)."""
     c = Config()
     load().setmhd_config_cartesian(ctypes.byref(c), float(cfl), float(r_hold))
@@ -111,6 +112,7 @@ def cartesian_config(cfl=0.4, r_hold=0.1, precision=PREC_FP64, time_scheme=TIME_
     c.time_scheme = time_scheme
     c.tile_x, c.tile_y = tile
     c.device = device
+    c.kchunk = kchunk
     return c
 
 

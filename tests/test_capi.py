@@ -36,4 +36,4 @@ def test_config_defaults_without_device():
     assert ref.cfl == float(__import__("numpy").float32(0.2))
     car = solver.cartesian_config(cfl=0.4, r_hold=0.1)
     assert (car.riemann, car.recon, car.time_scheme, car.flags) == (3, 0, 0, 0)
-    assert car.cfl == 0.8 and car.r_hold_cell == 0.1
+    assert car.cfl == 0.4 and car.r_hold_cell == 0.1

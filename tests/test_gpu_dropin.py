@@ -136,3 +136,41 @@ def test_upload_validation():
         W2[5, 0] = 0.0
         with pytest.raises(solver.SetMHDError):
             c.upload_mesh(fm, W2)
+
+
+@pytest.mark.parametrize("mode", ["reference", "cartesian"])
+def test_fortran_driver(tmp_path, mode):
+    """The Fortran entry point (set_amd/fortran/fcmhd_driver, iso_c_binding
+    to the C ABI) reads an FCMHD input file, runs the steps on the GPU and
+    writes the Save_Storage file; compared with the oracle from dt = 2e-9."""
+    import os
+    import subprocess
+
+    from set_amd import fcmhd
+    case(f"fortran driver {mode}")
+    exe = os.path.join(os.path.dirname(solver.HERE), "set_amd", "fortran", "fcmhd_driver")
+    if mode == "reference":
+        fm, W = helio_mesh(60, 30)
+        cfg = oracle.reference_config()
+    else:
+        g = mesh.grid_2d(60, 30)
+        fm = mesh.face_mesh(g)
+        W = mesh.heliosphere_state(g)
+        cfg = oracle.cartesian_config()
+    fin, fout = tmp_path / "FCMHD_3.bin", tmp_path / "FCMHD_3.5_out.bin"
+    fcmhd.write_input(fin, fm, W)
+    args = [exe, str(fin), str(fout), "12", "5", ""] + (["cartesian"] if mode == "cartesian" else [])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    t, cp = fcmhd.fill_data(fout, fm.n_cell)
+    s = oracle.Solver(fm, W, cfg)
+    s.dt_next = 2.0e-9
+    assert s.run(12) == 0
+    if mode == "cartesian":
+        np.testing.assert_array_equal(cp[:, :8], s.W[:, :8])
+    else:
+        err = np.abs(cp[:, :8] - s.W[:, :8]).max(0) / np.maximum(np.abs(s.W[:, :8]).max(0), 1e-300)
+        assert np.all(err <= 1e-12), err
+    np.testing.assert_array_equal(cp[:, 8:], W[:, 8:])
+    assert t == pytest.approx(s.t, rel=1e-13)
