@@ -101,7 +101,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calib_copy:
+        import torch
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes": a.numel() * 8}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)

@@ -75,4 +75,15 @@ __device__ __forceinline__ double fsqrt(double x) {
 }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_sqrtf(x); }
 
+// 1/sqrt of a positive normal number: estimate + two Newton steps (FP64).
+__device__ __forceinline__ double frsq(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double hx = 0.5 * x;
+  double e = __builtin_fma(-hx * y, y, 0.5);
+  y = __builtin_fma(y, e, y);
+  e = __builtin_fma(-hx * y, y, 0.5);
+  return __builtin_fma(y, e, y);
+}
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
 }  // namespace setmhd

@@ -77,10 +77,6 @@ __device__ __forceinline__ bool hlld(const Side<R>& L, const Side<R>& Rr, R F[8]
     F[5] = fbn;
     return true;
   }
-  R FL[8], FR[8];
-  phys_flux(L, FL);
-  phys_flux(Rr, FR);
-
   const R suR = SR - Rr.vn, suL = SL - L.vn;
   const R mR = suR * Rr.rho, mL = suL * L.rho;
   const R idn = frcp(mR - mL);
@@ -121,9 +117,10 @@ __device__ __forceinline__ bool hlld(const Side<R>& L, const Side<R>& Rr, R F[8]
   // double-star states (:1658-1698)
   const R abn = fabs(bn);
   const bool ibn = abn > R(1e-6);
-  const R sqL = fsqrt(rzL), sqR = fsqrt(rzR);
-  const R SZL = ibn ? SM - abn * frcp(sqL) : SM;
-  const R SZR = ibn ? SM + abn * frcp(sqR) : SM;
+  const R irL = frsq(rzL), irR = frsq(rzR);  // 1/sqrt(rho*)
+  const R sqL = rzL * irL, sqR = rzR * irR;
+  const R SZL = ibn ? SM - abn * irL : SM;
+  const R SZR = ibn ? SM + abn * irR : SM;
 
   // region, sequential tests of :1722-1814 (later wins)
   int j = 0;
@@ -139,21 +136,25 @@ __device__ __forceinline__ bool hlld(const Side<R>& L, const Side<R>& Rr, R F[8]
   const R vz1 = left ? vL1 : vR1, vz2 = left ? vL2 : vR2;
   const R bz1 = left ? bL1 : bR1, bz2 = left ? bL2 : bR2;
   const R ez = left ? ezL : ezR;
-  const R qr = left ? L.rho : Rr.rho, qvn = left ? L.vn : Rr.vn;
-  const R qv1 = left ? L.vt1 : Rr.vt1, qv2 = left ? L.vt2 : Rr.vt2;
-  const R qe = left ? L.e : Rr.e, qbn = left ? L.bn : Rr.bn;
-  const R qb1 = left ? L.bt1 : Rr.bt1, qb2 = left ? L.bt2 : Rr.bt2;
-  R dz[8];  // U* - U
-  dz[0] = rz - qr;
-  dz[1] = rz * SM - qr * qvn;
-  dz[2] = rz * vz1 - qr * qv1;
-  dz[3] = rz * vz2 - qr * qv2;
-  dz[4] = ez - qe;
-  dz[5] = bn - qbn;
-  dz[6] = bz1 - qb1;
-  dz[7] = bz2 - qb2;
-#pragma unroll
-  for (int k = 0; k < 8; k++) F[k] = (left ? FL[k] : FR[k]) + S * dz[k];
+  Side<R> Q;  // the side whose fan holds the face
+  Q.rho = left ? L.rho : Rr.rho;
+  Q.vn = left ? L.vn : Rr.vn;
+  Q.vt1 = left ? L.vt1 : Rr.vt1;
+  Q.vt2 = left ? L.vt2 : Rr.vt2;
+  Q.e = left ? L.e : Rr.e;
+  Q.bn = left ? L.bn : Rr.bn;
+  Q.bt1 = left ? L.bt1 : Rr.bt1;
+  Q.bt2 = left ? L.bt2 : Rr.bt2;
+  Q.pt = left ? L.pt : Rr.pt;
+  phys_flux(Q, F);  // F_L or F_R
+  // F* = F_Q + S (U* - U_Q)
+  F[0] += S * (rz - Q.rho);
+  F[1] += S * (rz * SM - Q.rho * Q.vn);
+  F[2] += S * (rz * vz1 - Q.rho * Q.vt1);
+  F[3] += S * (rz * vz2 - Q.rho * Q.vt2);
+  F[4] += S * (ez - Q.e);
+  F[6] += S * (bz1 - Q.bt1);
+  F[7] += S * (bz2 - Q.bt2);
 
   if (j == 3 || j == 4) {  // U** - U*, weighted by SZL / SZR
     const R sbn = bn > R(0) ? R(1) : R(-1);

@@ -102,22 +102,56 @@ __device__ __forceinline__ void put_q(Smem<R, TX, TY>& sm, int ly, int lx, const
   sm.q[QCF + 1][ly][lx] = q.cf[1];
 }
 
-// face-frame side of the LDS cell (ly, lx) for an x- (d = 0) or y- (d = 1) face
-template <class R, int TX, int TY>
-__device__ __forceinline__ Side<R> get_side(const Smem<R, TX, TY>& sm, int ly, int lx, int d) {
-  const int d1 = d + 1, d2 = d == 0 ? 2 : 0;
+// face-frame side of the LDS cell (ly, lx) for an x- (D = 0) or y- (D = 1) face
+template <int D, class R, int TX, int TY>
+__device__ __forceinline__ Side<R> get_side(const Smem<R, TX, TY>& sm, int ly, int lx) {
+  constexpr int D1 = D + 1, D2 = D == 0 ? 2 : 0;
   Side<R> s;
   s.rho = sm.q[QR][ly][lx];
-  s.vn = sm.q[QU + d][ly][lx];
-  s.vt1 = sm.q[QU + d1][ly][lx];
-  s.vt2 = sm.q[QU + d2][ly][lx];
-  s.bn = sm.q[QB + d][ly][lx];
-  s.bt1 = sm.q[QB + d1][ly][lx];
-  s.bt2 = sm.q[QB + d2][ly][lx];
+  s.vn = sm.q[QU + D][ly][lx];
+  s.vt1 = sm.q[QU + D1][ly][lx];
+  s.vt2 = sm.q[QU + D2][ly][lx];
+  s.bn = sm.q[QB + D][ly][lx];
+  s.bt1 = sm.q[QB + D1][ly][lx];
+  s.bt2 = sm.q[QB + D2][ly][lx];
   s.pt = sm.q[QPT][ly][lx];
   s.e = sm.q[QE][ly][lx];
-  s.cf = sm.q[QCF + d][ly][lx];
+  s.cf = sm.q[QCF + D][ly][lx];
   return s;
+}
+
+// One x- (D = 0) or y- (D = 1) face between LDS cells (ly, lx) and
+// (ry, rx); its flux, in x,y,z components, goes to the LDS flux slot of the
+// face (the face on the low side of cell (ry, rx)).
+template <int D, class R, int TX, int TY>
+__device__ __forceinline__ void lds_face(Smem<R, TX, TY>& sm, int ly, int lx, int ry, int rx,
+                                         double& smax, bool& ok) {
+  constexpr int D1 = D + 1, D2 = D == 0 ? 2 : 0;
+  const Side<R> L = get_side<D>(sm, ly, lx);
+  const Side<R> Rr = get_side<D>(sm, ry, rx);
+  R F[8], s;
+  ok &= hlld(L, Rr, F, s);
+  smax = fmax(smax, (double)s);
+  auto& dst = [&]() -> auto& {
+    if constexpr (D == 0) return sm.fx;
+    else return sm.fy;
+  }();
+  const int fy = ry - 1, fx = rx - 1;
+  dst[0][fy][fx] = F[0];
+  dst[1 + D][fy][fx] = F[1];
+  dst[1 + D1][fy][fx] = F[2];
+  dst[1 + D2][fy][fx] = F[3];
+  dst[4][fy][fx] = F[4];
+  dst[5 + D][fy][fx] = F[5];
+  dst[5 + D1][fy][fx] = F[6];
+  dst[5 + D2][fy][fx] = F[7];
+}
+
+// The 8 fields of cell idx: uniform (SGPR) field base + 32-bit offset.
+__device__ __forceinline__ void load8(const double* __restrict__ base, long long nc, unsigned idx,
+                                      double U[8]) {
+#pragma unroll
+  for (int v = 0; v < 8; v++) U[v] = (base + v * nc)[idx];
 }
 
 __device__ __forceinline__ bool held_cell(const SGrid& g, double xc, double yc, double zc) {
@@ -128,15 +162,19 @@ __device__ __forceinline__ bool held_cell(const SGrid& g, double xc, double yc, 
   return __dsqrt_rn(r2) < g.r_hold;
 }
 
+// Applies the face sum acc to cell idx.  The cell's own state is read back
+// from the input (an L2 hit: it was read one plane earlier) rather than kept
+// live in registers across the march.
 template <int MODE>
-__device__ __forceinline__ void finalize(const FusedArgs& a, double dt, long long idx, bool held,
-                                         const double Uold[8], const double acc[8],
-                                         double& rs_d, double& rs_r) {
+__device__ __forceinline__ void finalize(const FusedArgs& a, double dt, unsigned idx, bool held,
+                                         const double acc[8], double& rs_d, double& rs_r) {
   const SGrid& g = a.g;
+  double Uold[8];
+  load8(a.Uf, g.nc, idx, Uold);
   if (held) {
     if (MODE != kModeHeun2) {
 #pragma unroll
-      for (int v = 0; v < 8; v++) a.Uo[v * g.nc + idx] = Uold[v];
+      for (int v = 0; v < 8; v++) (a.Uo + v * g.nc)[idx] = Uold[v];
     }
     return;
   }
@@ -145,7 +183,7 @@ __device__ __forceinline__ void finalize(const FusedArgs& a, double dt, long lon
   if (MODE == kModeHeun2) {
 #pragma unroll
     for (int v = 0; v < 8; v++) {
-      double un = a.Un[v * g.nc + idx];
+      double un = (a.Un + v * g.nc)[idx];
       Un[v] = 0.5 * (un + Uold[v] - f * acc[v]);
       if (v == 0) {
         rs_d += fabs(Un[0] - un);
@@ -159,19 +197,21 @@ __device__ __forceinline__ void finalize(const FusedArgs& a, double dt, long lon
     rs_r += fabs(Uold[0]);
   }
   if (!(Un[0] > 0.0)) atomicOr(a.err, kErrRho);  // cuf_solvers.cuf:708-711
-  // p <= 0 -> 1e-7 (cuf_solvers.cuf:777-779), applied to the total energy
-  const double ir = frcp(Un[0]);
-  const double ke = 0.5 * (Un[1] * Un[1] + Un[2] * Un[2] + Un[3] * Un[3]) * ir;
-  const double me = 0.5 * (Un[5] * Un[5] + Un[6] * Un[6] + Un[7] * Un[7]);
-  if ((kGamma - 1.0) * (Un[4] - ke - me) <= 0.0)
-    Un[4] = SETMHD_F4(0.0000001) / (kGamma - 1.0) + ke + me;
+  // p <= 0 -> 1e-7 (cuf_solvers.cuf:777-779), applied to the total energy.
+  // For rho > 0, p > 0 <=> 2 E rho - |m|^2 - rho |b|^2 > 0: no division on
+  // the common path.
+  const double m2 = Un[1] * Un[1] + Un[2] * Un[2] + Un[3] * Un[3];
+  const double b2 = Un[5] * Un[5] + Un[6] * Un[6] + Un[7] * Un[7];
+  if (!(2.0 * Un[4] * Un[0] - m2 - Un[0] * b2 > 0.0))
+    Un[4] = SETMHD_F4(0.0000001) / (kGamma - 1.0) + 0.5 * m2 * frcp(Un[0]) + 0.5 * b2;
 #pragma unroll
-  for (int v = 0; v < 8; v++) a.Uo[v * g.nc + idx] = Un[v];
+  for (int v = 0; v < 8; v++) (a.Uo + v * g.nc)[idx] = Un[v];
 }
 
 template <class R, int TX, int TY, int DIM, int MODE>
-__global__ __launch_bounds__(TX* TY) void k_fused(FusedArgs a) {
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))) void k_fused(FusedArgs a) {
   constexpr int NH = 2 * TX + 2 * TY;
+  static_assert(TX <= 64 && 64 + TY <= TX * TY && NH <= TX * TY, "edge-face lane layout");
   __shared__ Smem<R, TX, TY> sm;
   __shared__ double red[TX * TY / 64];
   const SGrid& g = a.g;
@@ -189,7 +229,8 @@ __global__ __launch_bounds__(TX* TY) void k_fused(FusedArgs a) {
   const int ic = min(i, g.nx - 1), jc = min(j, g.ny - 1);
   const int k0 = DIM == 3 ? kc * a.kchunk : 0;
   const int k1 = DIM == 3 ? min(k0 + a.kchunk, g.nz) : 1;
-  const long long plane = (long long)g.nx * g.ny;
+  const unsigned plane = (unsigned)g.nx * (unsigned)g.ny;
+  const unsigned col0 = (unsigned)ic + (unsigned)g.nx * (unsigned)jc;
   const double dt = a.dt.slot[a.cur];
   if (MODE != kModeHeun2 && blockIdx.x == 0 && tid == 0) {
     a.dt.t[0] = a.dt.t[0] + dt;                 // cuf_kernel.cuf:792-794
@@ -212,72 +253,73 @@ __global__ __launch_bounds__(TX* TY) void k_fused(FusedArgs a) {
 
   double smax[3] = {0.0, 0.0, 0.0};
   double rs_d = 0.0, rs_r = 0.0;
-  double Up[8], accp[8];
+  double accp[8];
   Side<R> zp;
   bool ok = true;
 
   if (DIM == 3) {  // z-face data of the plane below the chunk
     const int kb = max(k0 - 1, 0);
-    const long long idx = ic + (long long)g.nx * jc + plane * kb;
     double U[8];
-#pragma unroll
-    for (int v = 0; v < 8; v++) U[v] = a.Uf[v * g.nc + idx];
+    load8(a.Uf, g.nc, col0 + plane * kb, U);
     CellQ<R> q;
     derive<R, true>(U, q);
     zp = side_of(q, 2);
   }
 
+  // Software pipeline (FP64): the next plane's own and halo cells are loaded
+  // while the current plane's faces are formed.  The mixed form skips it:
+  // its 32 extra VGPRs would cost a wave per SIMD.
+  constexpr bool PF = sizeof(R) == 8;
+  const unsigned hcol = (unsigned)hgi + (unsigned)g.nx * (unsigned)hgj;
+  double Unx[8], Hnx[8];
+  if constexpr (PF) {
+    load8(a.Uf, g.nc, col0 + plane * k0, Unx);
+    if (has_h) load8(a.Uf, g.nc, hcol + plane * k0, Hnx);
+  }
+
   for (int k = k0; k < k1; ++k) {
-    const long long idx = ic + (long long)g.nx * jc + plane * k;
-    double Uc[8];
+    const unsigned idx = col0 + plane * k;
+    double Uc[8], Uh[8];
+    if constexpr (PF) {
 #pragma unroll
-    for (int v = 0; v < 8; v++) Uc[v] = a.Uf[v * g.nc + idx];
+      for (int v = 0; v < 8; v++) {
+        Uc[v] = Unx[v];
+        Uh[v] = Hnx[v];
+      }
+      if (DIM == 3) {  // plane k+1 (the epilogue's plane when k+1 == k1)
+        const int kn = min(k + 1, g.nz - 1);
+        load8(a.Uf, g.nc, col0 + plane * kn, Unx);
+        if (has_h && k + 1 < k1) load8(a.Uf, g.nc, hcol + plane * kn, Hnx);
+      }
+    } else {
+      load8(a.Uf, g.nc, idx, Uc);
+      if (has_h) load8(a.Uf, g.nc, hcol + plane * k, Uh);
+    }
     CellQ<R> qc;
     derive<R, DIM == 3>(Uc, qc);
     put_q(sm, ty + 1, tx + 1, qc);
     if (has_h) {
-      const long long hidx = hgi + (long long)g.nx * hgj + plane * k;
-      double Uh[8];
-#pragma unroll
-      for (int v = 0; v < 8; v++) Uh[v] = a.Uf[v * g.nc + hidx];
       CellQ<R> qh;
       derive<R, false>(Uh, qh);
       put_q(sm, hly, hlx, qh);
     }
     __syncthreads();
 
-    // x- and y-faces of the plane (+ far-edge faces on the first lanes)
-    const int ntask = tid < TX + TY ? 3 : 2;
-    for (int t = 0; t < ntask; t++) {
-      int d, ly, lx, ry, rx;
-      if (t == 0) { d = 0; ly = ty + 1; lx = tx; ry = ty + 1; rx = tx + 1; }
-      else if (t == 1) { d = 1; ly = ty; lx = tx + 1; ry = ty + 1; rx = tx + 1; }
-      else if (tid < TY) { d = 0; ly = tid + 1; lx = TX; ry = tid + 1; rx = TX + 1; }
-      else { d = 1; ly = TY; lx = tid - TY + 1; ry = TY + 1; rx = tid - TY + 1; }
-      const Side<R> L = get_side(sm, ly, lx, d);
-      const Side<R> Rr = get_side(sm, ry, rx, d);
-      R F[8], s;
-      ok &= hlld(L, Rr, F, s);
-      smax[d] = fmax(smax[d], (double)s);
-      const int d1 = d + 1, d2 = d == 0 ? 2 : 0;
-      // face-frame components -> x,y,z
-      R Fx[8];
-      Fx[0] = F[0];
-      Fx[1 + d] = F[1];
-      Fx[1 + d1] = F[2];
-      Fx[1 + d2] = F[3];
-      Fx[4] = F[4];
-      Fx[5 + d] = F[5];
-      Fx[5 + d1] = F[6];
-      Fx[5 + d2] = F[7];
-      if (d == 0) {
-        const int fy = ry - 1, fxi = rx - 1;  // face index: left of LDS col rx
-#pragma unroll
-        for (int v = 0; v < 8; v++) sm.fx[v][fy][fxi] = Fx[v];
-      } else {
-        const int fy = ry - 1, fxi = rx - 1;  // face below LDS row ry
-#pragma unroll
-        for (int v = 0; v < 8; v++) sm.fy[v][fy][fxi] = Fx[v];
+    // x- and y-faces of the plane.  The TY faces on the tile's far x edge
+    // go to lanes 64..64+TY-1 (wave 1), the TX faces on the far y edge to
+    // lanes 0..TX-1 (wave 0), each as a second iteration of its loop.
+    {
+      const int nx_it = (tid >= 64 && tid < 64 + TY) ? 2 : 1;
+      for (int r = 0; r < nx_it; r++) {
+        const int row = r == 0 ? ty + 1 : tid - 64 + 1;
+        const int col = r == 0 ? tx : TX;
+        lds_face<0>(sm, row, col, row, col + 1, smax[0], ok);
+      }
+      const int ny_it = tid < TX ? 2 : 1;
+      for (int r = 0; r < ny_it; r++) {
+        const int row = r == 0 ? ty : TY;
+        const int col = r == 0 ? tx + 1 : tid + 1;
+        lds_face<1>(sm, row, col, row + 1, col, smax[1], ok);
       }
     }
 
@@ -318,27 +360,20 @@ __global__ __launch_bounds__(TX* TY) void k_fused(FusedArgs a) {
 #pragma unroll
         for (int v = 0; v < 8; v++) accp[v] = accp[v] + fz[v] * g.az;
         const double zcen = __dadd_rn(g.z0, __dmul_rn((double)(k - 1) + 0.5, g.dz));
-        finalize<MODE>(a, dt, idx - plane, held_cell(g, xc, yc, zcen), Up, accp, rs_d, rs_r);
+        finalize<MODE>(a, dt, idx - plane, held_cell(g, xc, yc, zcen), accp, rs_d, rs_r);
       }
 #pragma unroll
-      for (int v = 0; v < 8; v++) {
-        Up[v] = Uc[v];
-        accp[v] = acc[v];
-      }
+      for (int v = 0; v < 8; v++) accp[v] = acc[v];
       zp = zc;
     } else if (own) {
-      finalize<MODE>(a, dt, idx, held_cell(g, xc, yc, 0.0), Uc, acc, rs_d, rs_r);
+      finalize<MODE>(a, dt, idx, held_cell(g, xc, yc, 0.0), acc, rs_d, rs_r);
     }
   }
 
   if (DIM == 3) {  // z-face above the chunk's last plane
-    const int ke = min(k1, g.nz - 1);
-    const long long idx = ic + (long long)g.nx * jc + plane * ke;
-    double U[8];
-#pragma unroll
-    for (int v = 0; v < 8; v++) U[v] = a.Uf[v * g.nc + idx];
+    if constexpr (!PF) load8(a.Uf, g.nc, col0 + plane * min(k1, g.nz - 1), Unx);
     CellQ<R> q;
-    derive<R, true>(U, q);
+    derive<R, true>(Unx, q);  // plane min(k1, nz-1)
     const Side<R> zc = side_of(q, 2);
     R Fz[8], s;
     ok &= hlld(zp, zc, Fz, s);
@@ -356,8 +391,8 @@ __global__ __launch_bounds__(TX* TY) void k_fused(FusedArgs a) {
 #pragma unroll
       for (int v = 0; v < 8; v++) accp[v] = accp[v] + fz[v] * g.az;
       const double zcen = __dadd_rn(g.z0, __dmul_rn((double)(k1 - 1) + 0.5, g.dz));
-      finalize<MODE>(a, dt, ic + (long long)g.nx * jc + plane * (k1 - 1),
-                     held_cell(g, xc, yc, zcen), Up, accp, rs_d, rs_r);
+      finalize<MODE>(a, dt, col0 + plane * (k1 - 1),
+                     held_cell(g, xc, yc, zcen), accp, rs_d, rs_r);
     }
   }
   if (!ok) atomicOr(a.err, kErrRegion);
