@@ -255,3 +255,28 @@ def test_source_mk_closed_form():
                                     (uh ** 2 + vh ** 2 - u ** 2 - v ** 2) / 2
                                     + uu / um * (2 * ph / rh - p / rho)])
         np.testing.assert_allclose(oracle.source_mk(cp), S, rtol=1e-12)
+
+
+def test_source_mk_vanishes_for_comoving_equal_temperature():
+    """Neutrals moving with the plasma at its temperature (2 pH/rhoH = p/rho,
+    pH = T rhoH / 2) exchange no momentum or energy."""
+    rng = np.random.default_rng(19)
+    for _ in range(50):
+        rho, u, v, p = rng.uniform(0.1, 2), rng.normal(), rng.normal(), rng.uniform(0.1, 2)
+        T = p / rho
+        cp = np.r_[rho, u, v, 0.0, p, 0, 0, 0, rng.uniform(0.01, 1), u, v, T,
+                   rng.uniform(0.01, 1), u, v, T]
+        S = oracle.source_mk(cp)
+        scale = rho * (1 + u * u + v * v + T)
+        assert np.all(np.abs(S) <= 1e-12 * scale), S
+
+
+def test_source_mk_linear_in_neutral_density():
+    rng = np.random.default_rng(20)
+    for _ in range(50):
+        cp = np.r_[rng.uniform(0.1, 2), rng.normal(size=3), rng.uniform(0.1, 2), np.zeros(3),
+                   rng.uniform(0.01, 1), rng.normal(size=2), rng.uniform(0.1, 2),
+                   rng.uniform(0.01, 1), rng.normal(size=2), rng.uniform(0.1, 2)]
+        c2 = cp.copy()
+        c2[[8, 12]] *= 2.0
+        np.testing.assert_allclose(oracle.source_mk(c2), 2.0 * oracle.source_mk(cp), rtol=1e-13)

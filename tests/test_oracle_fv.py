@@ -136,3 +136,74 @@ def test_lagged_dt_trace():
     assert s.dt_next == dt0
     s.run(1)
     assert s.t == 1.0e-5 + dt0
+
+
+def test_heun_is_average_of_two_euler_passes():
+    """Heun (cuf_solvers.cuf:740-772): U^{n+1} = (U^n + E(E(U^n)))/2 in
+    conserved variables, E = one forward-Euler face + cell pass with the
+    same dt."""
+    g = mesh.Grid(40, 20)
+    fm = mesh.face_mesh(g)
+    W = mesh.heliosphere_state(g)
+    cfg_e = oracle.cartesian_config()
+    dt = oracle.Solver(fm, W, cfg_e).dt_scan()
+    h = oracle.Solver(fm, W, oracle.cartesian_config(time_scheme=1))
+    h.dt_next = dt
+    assert h.run(1) == 0
+    e1 = oracle.Solver(fm, W, cfg_e)
+    e1.dt_next = dt
+    assert e1.run(1) == 0
+    e2 = oracle.Solver(fm, e1.W, cfg_e)
+    e2.dt_next = dt
+    assert e2.run(1) == 0
+    expect = 0.5 * (mesh.prim_to_cons(W[:, :8]) + mesh.prim_to_cons(e2.W[:, :8]))
+    got = mesh.prim_to_cons(h.W[:, :8])
+    held = np.linalg.norm(fm.cell_center, axis=1) < 0.1
+    np.testing.assert_allclose(got[~held], expect[~held], rtol=1e-12, atol=1e-14)
+
+
+def test_axisymmetric_sources_closed_form():
+    """Uniform state, axisymmetric sources on (cuf_solvers.cuf:706-736): the
+    face sum vanishes, so one Euler step is rho' = rho (1 - dt v/y), u and v
+    unchanged, E' = E - dt v (gamma p/(gamma-1) + rho (u^2+v^2)/2) / y."""
+    g = mesh.Grid(12, 10, 1, 0.0, 1.0, 1.0, 2.0)
+    fm = mesh.face_mesh(g)
+    rho, u, v, p = 0.8, 0.3, 0.2, 0.5
+    W = np.zeros((fm.n_cell, 16))
+    W[:, [0, 1, 2, 4]] = [rho, u, v, p]
+    cfg = oracle.cartesian_config(riemann=2)
+    cfg.flags = oracle.F_AXISYM | oracle.F_ZERO_BW
+    cfg.r_hold_cell = -1.0
+    s = oracle.Solver(fm, W, cfg)
+    dt = 1e-3
+    s.dt_next = dt
+    assert s.run(1) == 0
+    y = fm.cell_center[:, 1]
+    gam = mesh.GAMMA
+    rho1 = rho * (1 - dt * v / y)
+    E1 = p / (gam - 1) + 0.5 * rho * (u * u + v * v) - dt * v * (gam * p / (gam - 1) + rho * (u * u + v * v) * 0.5) / y
+    p1 = (E1 - 0.5 * rho1 * (u * u + v * v)) * (gam - 1)
+    i = np.arange(fm.n_cell)
+    inner = ((i % 12) > 0) & ((i % 12) < 11) & ((i // 12) > 0) & ((i // 12) < 9)
+    np.testing.assert_allclose(s.W[inner, 0], rho1[inner], rtol=1e-13)
+    np.testing.assert_allclose(s.W[inner, 1], u, rtol=1e-13)
+    np.testing.assert_allclose(s.W[inner, 2], v, rtol=1e-13)
+    np.testing.assert_allclose(s.W[inner, 4], p1[inner], rtol=1e-12)
+
+
+def test_dt_scan_closed_form():
+    """Uniform state: dt = cfl * (h/2) / max over axes of (|u_n| + c_f)."""
+    g = mesh.grid_3d(6, 5, 4)
+    fm = mesh.face_mesh(g)
+    q = np.array([0.7, 0.9, -0.4, 0.2, 0.6, 0.5, -0.3, 0.8])
+    W = np.zeros((fm.n_cell, 16))
+    W[:, :8] = q
+    cfg = oracle.cartesian_config(cfl=0.4, r_hold=-1.0)
+    dt = oracle.Solver(fm, W, cfg).dt_scan()
+    b = q[5:8] / mesh.SPI4
+    c2, a2 = mesh.GAMMA * q[4] / q[0], b @ b / q[0]
+    best = np.inf
+    for ax, h in enumerate((g.dx, g.dy, g.dz)):
+        cf = np.sqrt(0.5 * (c2 + a2 + np.sqrt((c2 + a2) ** 2 - 4 * c2 * b[ax] ** 2 / q[0])))
+        best = min(best, 0.4 * (h / 2) / (abs(q[1 + ax]) + cf))
+    assert dt == pytest.approx(best, rel=1e-13)
