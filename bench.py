@@ -71,6 +71,49 @@ def run_form(g, W, precision, time_scheme, steps, warmup):
     }
 
 
+def batch_form(precision, steps, seq_steps):
+    """64 uneven 2-D grids in one persistent launch per step (graph-replayed,
+    autotuned tiles) against the same grids stepped one at a time."""
+    from set_amd import solver, workloads
+    grids = workloads.batch_grids(64)
+    cells = sum(g.n_cell for g, _ in grids)
+    with solver.Context(solver.cartesian_config(precision=precision)) as c:
+        slots = [c.batch_add(g, W) for g, W in grids]
+        tile = c.batch_autotune(probe_steps=5)
+        c.batch_step(10)
+        c.batch_query(slots[0])
+        t0 = time.perf_counter()
+        c.batch_step(steps)
+        c.batch_query(slots[0])
+        wall = time.perf_counter() - t0
+    seq = 0.0
+    for g, W in grids:
+        with solver.Context(solver.cartesian_config(precision=precision)) as c:
+            c.upload_structured(g, W)
+            c.dt_from_state()
+            c.step(3)
+            c.sync()
+            t0 = time.perf_counter()
+            c.step(seq_steps)
+            c.sync()
+            seq += (time.perf_counter() - t0) / seq_steps
+    value = cells * steps / wall
+    return {"value": value, "grids": len(grids), "cells": cells, "steps": steps,
+            "ms_per_step": 1e3 * wall / steps, "tile": list(tile),
+            "hbm_frac": value * 128 / (PEAK_HBM_GBS * 1e9),
+            "sequential_value": cells / seq, "speedup_vs_one_at_a_time": (cells / seq and value / (cells / seq))}
+
+
+def serving_form(n_requests, max_steps, precision=0):
+    from set_amd import serve, workloads
+    reqs = workloads.serving_requests(n_requests)
+    states = [r.state() for r in reqs]
+    _, b = serve.Server(precision=precision, max_steps=max_steps).run(reqs, states)
+    _, q = serve.run_sequential(reqs, states, precision=precision, max_steps=max_steps)
+    return {"requests": n_requests, "max_steps": max_steps, "batched": b, "one_at_a_time": q,
+            "solves_per_hour_ratio": b["solves_per_hour"] / q["solves_per_hour"]}
+
+
 def cpu_baseline(n, steps, threads):
     import oracle
     from set_amd import mesh
@@ -97,10 +140,13 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--forms", default="fp64,mixed,heun")
+    ap.add_argument("--forms", default="fp64,mixed,heun,batch64,serving")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-max-steps", type=int, default=5000)
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -135,6 +181,14 @@ def main():
     for f in forms.values():
         f["hbm_frac"] = f["achieved_GBs"] / PEAK_HBM_GBS
     del W
+    for name, fn in (("batch64_fp64", lambda: batch_form(0, args.batch_steps, 20)),
+                     ("batch64_mixed", lambda: batch_form(1, args.batch_steps, 20)),
+                     ("serving_fp64", lambda: serving_form(args.serve_requests, args.serve_max_steps))):
+        if name.split("_")[0] in wanted:
+            try:
+                forms[name] = fn()
+            except Exception as e:  # a failing extra form must not lose the headline line
+                forms[name] = {"error": repr(e)}
 
     traffic = traffic_from_profiles(f"fused_fp64_euler_{n}cube")
     achieved = fp["achieved_GBs"]

@@ -160,6 +160,48 @@ int setmhd_sizes(const setmhd_ctx *ctx, int64_t *n_cell, int64_t *n_gran);
  * ||rho^n||_1 (structured path; serving convergence test). */
 int setmhd_residual(setmhd_ctx *ctx, double *res);
 
+/* ---- batches of structured grids (continuous batching) -------------------
+ * A context can instead hold many independent structured grids ("slots").
+ * All admitted grids advance together: one persistent fused launch per step
+ * per dimensionality, captured in a HIP graph; each grid keeps its own
+ * lagged dt, time, step count and density residual on the device.  Grids
+ * may be admitted and retired between calls.  Forward Euler only.
+ * No counterpart in the reference (its solver holds one mesh); this is the
+ * batching / serving form of the same step. */
+
+/* Admits a grid (as setmhd_upload_structured) with its CFL-sized first dt;
+ * *slot receives its id. */
+int setmhd_batch_add(setmhd_ctx *ctx, int32_t nx, int32_t ny, int32_t nz,
+                     const double *box /*[6]*/, const double *cell_par,
+                     double t0, int32_t *slot);
+
+/* Retires a grid and frees its device memory. */
+int setmhd_batch_remove(setmhd_ctx *ctx, int32_t slot);
+
+/* Advances every admitted grid by nsteps steps. */
+int setmhd_batch_step(setmhd_ctx *ctx, int32_t nsteps);
+
+/* Steps since admission, time, next dt and ||rho^{n+1}-rho^n||_1/||rho^n||_1
+ * of the slot's last step (any pointer may be NULL). */
+int setmhd_batch_query(setmhd_ctx *ctx, int32_t slot, int64_t *steps,
+                       double *t, double *dt_next, double *residual);
+
+/* Cell_par of a slot (16 doubles per cell) and its time. */
+int setmhd_batch_download(setmhd_ctx *ctx, int32_t slot, double *cell_par,
+                          double *t);
+
+/* Times probe_steps steps of the admitted batch for each tile shape
+ * (32x8, 16x16, 64x4, 16x8) and persistent grid size (1, 2, 4, 8, 64
+ * blocks per CU), restores the state after each probe, keeps the fastest for this
+ * context and caches it per (precision, dimensionality, size bucket).
+ * Outputs may be NULL. */
+int setmhd_batch_autotune(setmhd_ctx *ctx, int32_t probe_steps,
+                          int32_t *tile_x, int32_t *tile_y,
+                          int32_t *blocks_per_cu);
+
+/* Free and total device memory (admission control). */
+int setmhd_mem_info(setmhd_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes);
+
 /* Message of the last failure of this context ("" if none). */
 int setmhd_last_error(const setmhd_ctx *ctx, char *buf, size_t n);
 

@@ -11,9 +11,13 @@
 // error word replaces the reference's in-kernel STOP.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/setmhd.h"
@@ -42,7 +46,46 @@ struct setmhd_ctx {
   double *d_U = nullptr, *d_U2 = nullptr;  // Euler: ping-pong; Heun: U^n, U^1
   int cur_buf = 0;                         // Euler: which buffer holds U^n
   double* d_H16 = nullptr;                 // Cell_par(9:16) kept for download
-  int kchunk = 32;
+  int kchunk = 32;  // z-planes per block of the single structured grid
+  // batch of structured grids (kind 3)
+  struct Slot {
+    bool used = false;
+    SGrid g{};
+    double* buf[2] = {nullptr, nullptr};
+    double* h16 = nullptr;
+    long long steps = 0;  // steps since admission (mirrors the device nstep)
+  };
+  std::vector<Slot> slots;
+  void* d_slot_scal = nullptr;  // SlotScal[cap]
+  int slot_cap = 0;
+  bool dirty = true;            // tables / graph need a rebuild
+  BatchGrid* d_bgrid[2] = {nullptr, nullptr};  // per dimensionality (2-D, 3-D)
+  int* d_tgrid[2] = {nullptr, nullptr};
+  int* d_tfirst[2] = {nullptr, nullptr};
+  int* d_counts[2] = {nullptr, nullptr};
+  BatchLaunch bl[2]{};
+  bool tables_dirty = true;     // tables need re-writing (no new graph)
+  std::vector<BatchGrid> stage_g[2];
+  std::vector<int> stage_t[2], stage_f[2];
+  int stage_n[2][2] = {{0, 0}, {0, 0}};
+  std::vector<struct SlotScal> scal_host;  // host mirror after each batch step
+  bool mirror_ok = false;
+  hipGraph_t graph1 = nullptr, graphN = nullptr;
+  hipGraphExec_t exec1 = nullptr, execN = nullptr;
+  int graph_steps = 8;
+  int bt_x = 32, bt_y = 8, bt_bpc = 2;  // batch tile shape, blocks per CU
+  int n_cu = 256;
+};
+
+// per-slot device scalars of a batch
+struct SlotScal {
+  double slot[3];
+  double t;
+  double resid[2];
+  double resid_last[2];
+  int nstep;
+  int done;
+  int pad[2];
 };
 
 namespace {
@@ -67,6 +110,39 @@ int dalloc(setmhd_ctx* c, T** p, size_t n) {
   return SETMHD_OK;
 }
 
+void free_graphs(setmhd_ctx* c) {
+  if (c->exec1) hipGraphExecDestroy(c->exec1);
+  if (c->execN) hipGraphExecDestroy(c->execN);
+  if (c->graph1) hipGraphDestroy(c->graph1);
+  if (c->graphN) hipGraphDestroy(c->graphN);
+  c->exec1 = c->execN = nullptr;
+  c->graph1 = c->graphN = nullptr;
+}
+
+void free_batch(setmhd_ctx* c) {
+  free_graphs(c);
+  for (auto& sl : c->slots) {
+    for (double* p : {sl.buf[0], sl.buf[1], sl.h16})
+      if (p) hipFree(p);
+  }
+  c->slots.clear();
+  for (int d = 0; d < 2; d++) {
+    for (void* p : {(void*)c->d_bgrid[d], (void*)c->d_tgrid[d], (void*)c->d_tfirst[d], (void*)c->d_counts[d]})
+      if (p) hipFree(p);
+    c->d_bgrid[d] = nullptr;
+    c->d_tgrid[d] = nullptr;
+    c->d_tfirst[d] = nullptr;
+    c->d_counts[d] = nullptr;
+    c->bl[d] = BatchLaunch{};
+  }
+  if (c->d_slot_scal) hipFree(c->d_slot_scal);
+  c->d_slot_scal = nullptr;
+  c->slot_cap = 0;
+  c->dirty = true;
+  c->tables_dirty = true;
+  c->mirror_ok = false;
+}
+
 void free_state(setmhd_ctx* c) {
   for (double** p : {&c->d_W, &c->d_W2, &c->d_H, &c->d_potok, &c->d_U, &c->d_U2, &c->d_H16}) {
     if (*p) hipFree(*p);
@@ -74,6 +150,7 @@ void free_state(setmhd_ctx* c) {
   }
   if (c->d_mesh) hipFree(c->d_mesh);
   c->d_mesh = nullptr;
+  free_batch(c);
   c->kind = 0;
 }
 
@@ -194,6 +271,18 @@ int upload_scalars(setmhd_ctx* c, double t0) {
 
 UCfg ucfg(const setmhd_config& k) {
   return UCfg{k.riemann, k.recon, k.flags, k.cfl, k.dt_cap, k.r_skip_face, k.r_hold_cell};
+}
+
+// z-planes per block of a 3-D grid: the configured value, else enough
+// blocks to fill the chip (4 per CU with 32x8 columns), at most 32 planes
+// and at least 8 (each chunk re-reads two boundary planes).  A grid gets the
+// same chunking alone and in a batch, so both compute it identically.
+int grid_kchunk(const setmhd_ctx* c, int nx, int ny, int nz) {
+  if (c->cfg.kchunk > 0) return c->cfg.kchunk;
+  if (nz <= 1) return 32;
+  const long long txy = (long long)((nx + 31) / 32) * ((ny + 7) / 8);
+  const long long kc = (nz * txy + 4LL * c->n_cu - 1) / (4LL * c->n_cu);
+  return (int)std::min(32LL, std::max(8LL, kc));
 }
 
 int tile_x(const setmhd_ctx* c) { return c->cfg.tile_x > 0 ? c->cfg.tile_x : 32; }
@@ -317,6 +406,9 @@ int setmhd_create(setmhd_ctx** out, const setmhd_config* cfg) {
   if (hipMalloc((void**)&c->d_scal, 8 * sizeof(double)) != hipSuccess) return bad(SETMHD_ERR_NOMEM);
   if (hipMalloc((void**)&c->d_err, sizeof(int)) != hipSuccess) return bad(SETMHD_ERR_NOMEM);
   if (upload_scalars(c, 0.0) != SETMHD_OK) return bad(SETMHD_ERR_HIP);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
+    c->n_cu = prop.multiProcessorCount;
   *out = c;
   return SETMHD_OK;
 }
@@ -484,6 +576,7 @@ int setmhd_upload_structured(setmhd_ctx* c, int32_t nx, int32_t ny, int32_t nz, 
   g.cfl = c->cfg.cfl;
   g.dt_cap = c->cfg.dt_cap;
   g.r_hold = c->cfg.r_hold_cell;
+  c->kchunk = grid_kchunk(c, nx, ny, nz);
   c->cur_buf = 0;
   c->n_cell = nc;
   c->n_gran = nz > 1 ? (long long)(nx + 1) * ny * nz + (long long)nx * (ny + 1) * nz +
@@ -508,11 +601,12 @@ int setmhd_dt_from_state(setmhd_ctx* c, double* dt_out) {
   // Euler pass into the spare buffer (its output is discarded).
   HIPCHK(c, hipStreamSynchronize(c->stream));
   double save[6];
-  HIPCHK(c, hipMemcpy(save, c->d_scal, sizeof save, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpyAsync(save, c->d_scal, sizeof save, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   const int cur = (int)(c->step % 3);
   double h[6] = {save[0], save[1], save[2], save[3], save[4], save[5]};
   h[(cur + 1) % 3] = kDtReset;
-  HIPCHK(c, hipMemcpy(c->d_scal, h, sizeof h, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpyAsync(c->d_scal, h, sizeof h, hipMemcpyHostToDevice, c->stream));
   hipError_t e;
   if (c->kind == 1) {
     e = launch_face_pass(c->stream, c->um, ucfg(c->cfg), c->d_W, c->d_potok, ring(c), cur, 1, 0,
@@ -538,12 +632,14 @@ int setmhd_dt_from_state(setmhd_ctx* c, double* dt_out) {
   if (e != hipSuccess) return fail(c, SETMHD_ERR_HIP, hipGetErrorString(e));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   double r[6];
-  HIPCHK(c, hipMemcpy(r, c->d_scal, sizeof r, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpyAsync(r, c->d_scal, sizeof r, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   double dt = r[(cur + 1) % 3];
   // restore the ring and time, then arm dt for the next step
   save[cur] = dt;
   save[(cur + 1) % 3] = kDtReset;
-  HIPCHK(c, hipMemcpy(c->d_scal, save, sizeof save, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpyAsync(c->d_scal, save, sizeof save, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   if (dt_out) *dt_out = dt;
   return check_err(c);
 }
@@ -646,3 +742,5 @@ int setmhd_last_error(const setmhd_ctx* c, char* buf, size_t n) {
 }
 
 }  // extern "C"
+
+#include "batch_impl.inc"

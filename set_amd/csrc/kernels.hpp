@@ -57,6 +57,37 @@ struct FusedArgs {
   int* err;
 };
 
+// One grid of a batch (device-resident; see k_fused_batch).
+struct BatchGrid {
+  SGrid g;
+  double* buf[2];     // ping-pong state buffers; U^n is buf[nstep & 1]
+  DtRing dt;          // this grid's dt slots and time
+  int* nstep;         // steps done since admission
+  int ntiles;         // tiles of this grid
+  int kchunk;
+  double* resid;      // [2] accumulators of the running step
+  double* resid_last; // [2] sums of the last finished step
+  int* err;
+};
+
+struct BatchLaunch {
+  const BatchGrid* grids;  // device array [cap_grids]
+  const int* tile_grid;    // [cap_tiles] grid of each tile
+  const int* tile_first;   // [cap_grids] first global tile of each grid
+  const int* counts;       // device [2]: tiles, grids in use
+  int cap_tiles, cap_grids;
+  int nblocks;             // persistent grid size
+  int dim;                 // 2 or 3: all grids of one launch share it
+};
+
+// tiles of grid g for a TX x TY tile (and kchunk planes per tile in 3-D)
+inline int tiles_of(const SGrid& g, int tx, int ty, int kchunk) {
+  const int nkz = g.nz > 1 ? (g.nz + kchunk - 1) / kchunk : 1;
+  return ((g.nx + tx - 1) / tx) * ((g.ny + ty - 1) / ty) * nkz;
+}
+
+hipError_t launch_batch(hipStream_t s, const BatchLaunch& L, int precision, int tile_x, int tile_y);
+
 // precision: 0 FP64, 1 mixed.  Returns hipErrorInvalidValue for an
 // unsupported tile shape.
 hipError_t launch_fused(hipStream_t s, const FusedArgs& a, int mode, int precision, int tile_x,

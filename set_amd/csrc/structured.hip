@@ -208,20 +208,19 @@ __device__ __forceinline__ void finalize(const FusedArgs& a, double dt, unsigned
   for (int v = 0; v < 8; v++) (a.Uo + v * g.nc)[idx] = Un[v];
 }
 
+// One tile (TX x TY cells, all planes of one z-chunk in 3-D) of one grid:
+// stage the planes, form the faces, update the cells, and fold the tile's
+// CFL candidate into the grid's next dt slot.  book: this tile also does the
+// step's bookkeeping (t += dt, re-arm the dt slot after next).
 template <class R, int TX, int TY, int DIM, int MODE>
-__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))) void k_fused(FusedArgs a) {
+__device__ __forceinline__ void fused_tile(const FusedArgs& a, int b, bool book,
+                                           Smem<R, TX, TY>& sm, double* red) {
   constexpr int NH = 2 * TX + 2 * TY;
   static_assert(TX <= 64 && 64 + TY <= TX * TY && NH <= TX * TY, "edge-face lane layout");
-  __shared__ Smem<R, TX, TY> sm;
-  __shared__ double red[TX * TY / 64];
   const SGrid& g = a.g;
   const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
 
-  // tile of this block; XCD-contiguous remap when the grid divides by 8
   const int ntx = (g.nx + TX - 1) / TX, nty = (g.ny + TY - 1) / TY;
-  int b = blockIdx.x;
-  const int nblk = gridDim.x;
-  if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
   const int tile_x = b % ntx, rest = b / ntx, tile_y = rest % nty, kc = rest / nty;
   const int i0 = tile_x * TX, j0 = tile_y * TY;
   const int i = i0 + tx, j = j0 + ty;
@@ -232,7 +231,7 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))) voi
   const unsigned plane = (unsigned)g.nx * (unsigned)g.ny;
   const unsigned col0 = (unsigned)ic + (unsigned)g.nx * (unsigned)jc;
   const double dt = a.dt.slot[a.cur];
-  if (MODE != kModeHeun2 && blockIdx.x == 0 && tid == 0) {
+  if (MODE != kModeHeun2 && book && tid == 0) {
     a.dt.t[0] = a.dt.t[0] + dt;                 // cuf_kernel.cuf:792-794
     a.dt.slot[(a.cur + 2) % 3] = kDtReset;      // :790, for step n+2
   }
@@ -422,6 +421,69 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))) voi
   }
 }
 
+// One grid: one block per tile, tiles remapped so each XCD (blockIdx % 8)
+// gets a contiguous range (neighbouring tiles share halo lines in its L2).
+template <class R, int TX, int TY, int DIM, int MODE>
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))) void k_fused(FusedArgs a) {
+  __shared__ Smem<R, TX, TY> sm;
+  __shared__ double red[TX * TY / 64];
+  int b = blockIdx.x;
+  const int nblk = gridDim.x;
+  if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
+  fused_tile<R, TX, TY, DIM, MODE>(a, b, blockIdx.x == 0, sm, red);
+}
+
+// Many grids, one persistent launch per step.  Block b walks a contiguous
+// range of the global tile list.  Each grid's step index lives on the
+// device (BatchGrid::nstep): it selects the dt slot and the ping-pong
+// buffer, so the launch has no per-step arguments and many steps can be
+// captured in one graph.  k_batch_book, launched after it on the same
+// stream, closes the step of every grid.
+template <class R, int TX, int TY, int DIM>
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))) void k_fused_batch(
+    const BatchGrid* __restrict__ grids, const int* __restrict__ tile_grid,
+    const int* __restrict__ tile_first, const int* __restrict__ counts) {
+  __shared__ Smem<R, TX, TY> sm;
+  __shared__ double red[TX * TY / 64];
+  const int ntiles = counts[0];  // on the device: the batch can change between graph replays
+  const int per = (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int t0 = (int)blockIdx.x * per, t1 = min(t0 + per, ntiles);
+  for (int t = t0; t < t1; t++) {
+    const int gi = tile_grid[t];
+    const BatchGrid& bg = grids[gi];
+    const int n = *bg.nstep;
+    FusedArgs a;
+    a.g = bg.g;
+    a.Uf = bg.buf[n & 1];
+    a.Un = a.Uf;
+    a.Uo = bg.buf[(n & 1) ^ 1];
+    a.dt = bg.dt;
+    a.cur = n % 3;
+    a.kchunk = bg.kchunk;
+    a.resid = bg.resid;
+    a.err = bg.err;
+    fused_tile<R, TX, TY, DIM, kModeEuler>(a, t - tile_first[gi], false, sm, red);
+    __syncthreads();  // LDS is reused by the next tile
+  }
+}
+
+// Closes one batch step for each grid: t += dt (cuf_kernel.cuf:792-794),
+// re-arms the dt slot after next (:790), publishes the step's density
+// residual and advances the grid's step index.
+__global__ void k_batch_book(const BatchGrid* __restrict__ grids, const int* __restrict__ counts) {
+  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= counts[1]) return;
+  const BatchGrid& bg = grids[gi];
+  const int n = *bg.nstep;
+  bg.dt.t[0] = bg.dt.t[0] + bg.dt.slot[n % 3];
+  bg.dt.slot[(n + 2) % 3] = kDtReset;
+  bg.resid_last[0] = bg.resid[0];
+  bg.resid_last[1] = bg.resid[1];
+  bg.resid[0] = 0.0;
+  bg.resid[1] = 0.0;
+  *bg.nstep = n + 1;
+}
+
 template <class R, int TX, int TY, int DIM, int MODE>
 static hipError_t launch_one(hipStream_t s, const FusedArgs& a) {
   const int ntx = (a.g.nx + TX - 1) / TX, nty = (a.g.ny + TY - 1) / TY;
@@ -451,6 +513,36 @@ static hipError_t launch_prec(hipStream_t s, const FusedArgs& a, int mode, int t
   if (tx == 64 && ty == 4) return launch_tile<R, 64, 4>(s, a, mode);
   if (tx == 16 && ty == 8) return launch_tile<R, 16, 8>(s, a, mode);
   return hipErrorInvalidValue;
+}
+
+template <class R, int TX, int TY>
+static hipError_t launch_batch_tile(hipStream_t s, const BatchLaunch& L) {
+  if (L.dim == 3)
+    hipLaunchKernelGGL((k_fused_batch<R, TX, TY, 3>), dim3(L.nblocks), dim3(TX * TY), 0, s, L.grids,
+                       L.tile_grid, L.tile_first, L.counts);
+  else
+    hipLaunchKernelGGL((k_fused_batch<R, TX, TY, 2>), dim3(L.nblocks), dim3(TX * TY), 0, s, L.grids,
+                       L.tile_grid, L.tile_first, L.counts);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_batch_book, dim3((L.cap_grids + 63) / 64), dim3(64), 0, s, L.grids, L.counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch(hipStream_t s, const BatchLaunch& L, int precision, int tile_x, int tile_y) {
+  if (L.nblocks <= 0 || L.cap_grids <= 0) return hipSuccess;
+#define SETMHD_BT(T)                                                         \
+  if (tile_x == 32 && tile_y == 8) return launch_batch_tile<T, 32, 8>(s, L);   \
+  if (tile_x == 16 && tile_y == 16) return launch_batch_tile<T, 16, 16>(s, L); \
+  if (tile_x == 64 && tile_y == 4) return launch_batch_tile<T, 64, 4>(s, L);   \
+  if (tile_x == 16 && tile_y == 8) return launch_batch_tile<T, 16, 8>(s, L);   \
+  return hipErrorInvalidValue;
+  if (precision == 0) {
+    SETMHD_BT(double)
+  } else {
+    SETMHD_BT(float)
+  }
+#undef SETMHD_BT
 }
 
 hipError_t launch_fused(hipStream_t s, const FusedArgs& a, int mode, int precision, int tile_x,

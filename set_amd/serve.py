@@ -1,0 +1,151 @@
+"""Continuous-batching server for heliosphere solve requests.
+
+Requests arrive over time (a seeded Poisson stream); each is a structured
+grid with its own inflow and field.  The server keeps every admitted
+request's state resident on the GPU as one slot of a batch context
+(setmhd_batch_*): all active requests advance together, one persistent
+fused launch per step per dimensionality, replayed from a captured graph.
+Between chunks of steps it
+
+  * admits arrived requests in arrival order while free HBM (hipMemGetInfo)
+    covers their state plus a reserve,
+  * checks each request's relative density residual
+    ||rho^{n+1} - rho^n||_1 / ||rho^n||_1 of its last step, and
+  * retires a request when the residual is <= tol or it reached max_steps,
+    downloading its state as the result.
+
+A request that arrives while others are on the GPU therefore starts at the
+next chunk boundary instead of waiting for the running ones to finish.
+run_sequential is the one-at-a-time baseline: each request is uploaded,
+solved alone with the single-grid path and downloaded before the next.
+"""
+import collections
+import time
+
+import numpy as np
+
+from . import solver
+
+# device bytes per cell of an admitted grid: two FP64 state buffers,
+# Cell_par(9:16) and the upload staging of Cell_par (16 doubles)
+BYTES_PER_CELL = 64 + 64 + 64 + 128
+
+
+def _latency_stats(done, wall):
+    lat = np.array([r["done"] - r["arrival"] for r in done])
+    return {
+        "solves": len(done),
+        "wall_s": wall,
+        "solves_per_hour": 3600.0 * len(done) / wall if wall > 0 else 0.0,
+        "latency_p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
+        "latency_p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
+        "converged": int(sum(r["converged"] for r in done)),
+        "cell_updates": float(sum(r["cells"] * r["steps"] for r in done)),
+    }
+
+
+class Server:
+    def __init__(self, precision=solver.PREC_FP64, check_every=50, max_steps=5000, tol=1e-6,
+                 reserve_bytes=4 << 30, max_active=512, autotune=True):
+        self.precision = precision
+        self.check_every = check_every
+        self.max_steps = max_steps
+        self.tol = tol
+        self.reserve = reserve_bytes
+        self.max_active = max_active
+        self.autotune = autotune
+
+    def run(self, requests, states, clock=time.perf_counter, sleep=time.sleep, keep_results=False):
+        """requests: workloads.Request list; states: their initial Cell_par
+        (same order).  Returns per-request records and summary stats."""
+        order = sorted(range(len(requests)), key=lambda i: requests[i].arrival)
+        pending = collections.deque(order)
+        active = {}  # slot -> [request index, steps]
+        done = []
+        tuned = not self.autotune
+        spent = collections.Counter()  # seconds per server activity
+        with solver.Context(solver.cartesian_config(precision=self.precision)) as ctx:
+            t0 = clock()
+            while pending or active:
+                now = clock() - t0
+                free, _ = ctx.mem_info()
+                while (pending and requests[pending[0]].arrival <= now
+                       and len(active) < self.max_active):
+                    i = pending[0]
+                    need = requests[i].cells * BYTES_PER_CELL
+                    if free - need < self.reserve:
+                        break  # admitted when memory frees up
+                    pending.popleft()
+                    ta = clock()
+                    slot = ctx.batch_add(requests[i].grid, states[i])
+                    spent["admit"] += clock() - ta
+                    active[slot] = [i, 0, clock() - t0]
+                    free -= need
+                if not active:
+                    sleep(max(0.0, requests[pending[0]].arrival - (clock() - t0)))
+                    continue
+                if not tuned and len(active) > 1:
+                    ta = clock()
+                    ctx.batch_autotune(probe_steps=2)
+                    spent["autotune"] += clock() - ta
+                    tuned = True
+                chunk = min(self.check_every,
+                            min(self.max_steps - a[1] for a in active.values()))
+                ta = clock()
+                ctx.batch_step(chunk)
+                spent["step"] += clock() - ta
+                for slot in list(active):
+                    i, steps, admitted = active[slot]
+                    steps += chunk
+                    q = ctx.batch_query(slot)
+                    conv = q["residual"] <= self.tol
+                    if conv or steps >= self.max_steps:
+                        ta = clock()
+                        cp, t = ctx.batch_download(slot)
+                        ctx.batch_remove(slot)
+                        spent["retire"] += clock() - ta
+                        del active[slot]
+                        done.append({"rid": requests[i].rid, "arrival": requests[i].arrival,
+                                     "admitted": admitted, "done": clock() - t0,
+                                     "steps": steps, "residual": q["residual"],
+                                     "converged": conv, "cells": requests[i].cells,
+                                     "t": t, "result": cp if keep_results else None})
+                    else:
+                        active[slot][1] = steps
+            wall = clock() - t0
+        stats = _latency_stats(done, wall)
+        stats["seconds_in"] = dict(spent)
+        return done, stats
+
+
+def run_sequential(requests, states, precision=solver.PREC_FP64, check_every=50, max_steps=5000,
+                   tol=1e-6, clock=time.perf_counter, sleep=time.sleep, keep_results=False):
+    """One request at a time, in arrival order, with the single-grid path."""
+    order = sorted(range(len(requests)), key=lambda i: requests[i].arrival)
+    done = []
+    t0 = clock()
+    for i in order:
+        r = requests[i]
+        wait = r.arrival - (clock() - t0)
+        if wait > 0:
+            sleep(wait)
+        admitted = clock() - t0
+        with solver.Context(solver.cartesian_config(precision=precision)) as c:
+            c.upload_structured(r.grid, states[i])
+            c.dt_from_state()
+            steps, res = 0, 1.0
+            while True:
+                n = min(check_every, max_steps - steps)
+                if n > 1:
+                    c.step(n - 1)
+                res = c.step_residual()
+                steps += n
+                if res <= tol or steps >= max_steps:
+                    break
+            cp, t = c.download()
+        done.append({"rid": r.rid, "arrival": r.arrival, "admitted": admitted,
+                     "done": clock() - t0, "steps": steps, "residual": res,
+                     "converged": res <= tol, "cells": r.cells, "t": t,
+                     "result": cp if keep_results else None})
+    wall = clock() - t0
+    return done, _latency_stats(done, wall)

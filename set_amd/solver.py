@@ -38,6 +38,8 @@ EXPORTS = (
     "setmhd_upload_mesh", "setmhd_upload_structured", "setmhd_set_dt", "setmhd_dt_from_state",
     "setmhd_step", "setmhd_step_timed", "setmhd_sync", "setmhd_download",
     "setmhd_download_potok", "setmhd_sizes", "setmhd_residual", "setmhd_last_error",
+    "setmhd_batch_add", "setmhd_batch_remove", "setmhd_batch_step", "setmhd_batch_query",
+    "setmhd_batch_download", "setmhd_batch_autotune", "setmhd_mem_info",
 )
 
 
@@ -87,6 +89,15 @@ def load():
     L.setmhd_sizes.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     L.setmhd_residual.argtypes = [P, PD]
     L.setmhd_last_error.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t]
+    I64P = ctypes.POINTER(ctypes.c_int64)
+    L.setmhd_batch_add.argtypes = [P, I32, I32, I32, PD, PD, D, ctypes.POINTER(I32)]
+    L.setmhd_batch_remove.argtypes = [P, I32]
+    L.setmhd_batch_step.argtypes = [P, I32]
+    L.setmhd_batch_query.argtypes = [P, I32, I64P, PD, PD, PD]
+    L.setmhd_batch_download.argtypes = [P, I32, PD, PD]
+    L.setmhd_batch_autotune.argtypes = [P, I32, ctypes.POINTER(I32), ctypes.POINTER(I32),
+                                        ctypes.POINTER(I32)]
+    L.setmhd_mem_info.argtypes = [P, I64P, I64P]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is not None and name not in ("setmhd_config_reference",
@@ -223,6 +234,49 @@ class Context:
         p = np.zeros((self.n_gran, 9))
         self._check(self._L.setmhd_download_potok(self._h, _pd(p)))
         return p
+
+    # -- batches of structured grids ---------------------------------------
+    def batch_add(self, grid, cell_par, t0=0.0):
+        """Admits a structured grid; returns its slot id."""
+        cp = np.ascontiguousarray(cell_par, np.float64)
+        assert cp.shape == (grid.n_cell, 16)
+        box = np.array([grid.x0, grid.x1, grid.y0, grid.y1, grid.z0, grid.z1], np.float64)
+        slot = I32()
+        self._check(self._L.setmhd_batch_add(self._h, grid.nx, grid.ny, grid.nz, _pd(box),
+                                             _pd(cp), float(t0), ctypes.byref(slot)))
+        self._slot_cells = getattr(self, "_slot_cells", {})
+        self._slot_cells[slot.value] = grid.n_cell
+        return slot.value
+
+    def batch_remove(self, slot):
+        self._check(self._L.setmhd_batch_remove(self._h, int(slot)))
+        self._slot_cells.pop(slot, None)
+
+    def batch_step(self, nsteps=1):
+        self._check(self._L.setmhd_batch_step(self._h, int(nsteps)))
+
+    def batch_query(self, slot):
+        n, t, dt, r = ctypes.c_int64(), D(), D(), D()
+        self._check(self._L.setmhd_batch_query(self._h, int(slot), ctypes.byref(n), ctypes.byref(t),
+                                               ctypes.byref(dt), ctypes.byref(r)))
+        return {"steps": n.value, "t": t.value, "dt": dt.value, "residual": r.value}
+
+    def batch_download(self, slot):
+        cp = np.zeros((self._slot_cells[slot], 16))
+        t = D()
+        self._check(self._L.setmhd_batch_download(self._h, int(slot), _pd(cp), ctypes.byref(t)))
+        return cp, t.value
+
+    def batch_autotune(self, probe_steps=10):
+        tx, ty, bpc = I32(), I32(), I32()
+        self._check(self._L.setmhd_batch_autotune(self._h, int(probe_steps), ctypes.byref(tx),
+                                                  ctypes.byref(ty), ctypes.byref(bpc)))
+        return tx.value, ty.value, bpc.value
+
+    def mem_info(self):
+        f, t = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self._L.setmhd_mem_info(self._h, ctypes.byref(f), ctypes.byref(t)))
+        return f.value, t.value
 
     def cuda_start_mgd(self, all_step, dt_init=2.0e-9):
         """CUDA_START_MGD (cuf_kernel.cuf:402-949) without the text dumps:

@@ -1,0 +1,109 @@
+"""Batches of structured grids (setmhd_batch_*): every grid in a batch
+evolves exactly as it does alone, whatever else is admitted or retired
+around it, and the autotuner leaves the state where it found it."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from set_amd import mesh, solver
+
+pytestmark = pytest.mark.gpu
+
+
+def case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def cons(cp):
+    return mesh.prim_to_cons(cp[:, :8])
+
+
+def alone(g, W, nsteps, precision=0):
+    with solver.Context(solver.cartesian_config(precision=precision)) as c:
+        c.upload_structured(g, W)
+        c.dt_from_state()
+        c.step(nsteps)
+        cp, t = c.download()
+    return cp, t
+
+
+def grids():
+    out = []
+    for k, (nx, ny, nz) in enumerate([(96, 48, 1), (130, 70, 1), (200, 100, 1), (40, 36, 34), (33, 17, 1)]):
+        g = mesh.Grid(nx, ny) if nz == 1 else mesh.grid_3d(nx, ny, nz)
+        W = mesh.heliosphere_state(g, inflow=mesh.Inflow(seed=100 + k, u=-1.5 - 0.4 * k,
+                                                        b=0.1 + 0.08 * k, tilt_deg=15 * k),
+                                   wind=mesh.Wind(r_in=0.2 if nz == 1 else 0.3))
+        out.append((g, W))
+    return out
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_equals_alone(precision):
+    case(f"batch of 5 (2-D + 3-D) vs alone, precision={precision}")
+    gs = grids()
+    with solver.Context(solver.cartesian_config(precision=precision)) as c:
+        slots = [c.batch_add(g, W) for g, W in gs]
+        c.batch_step(37)
+        res = [c.batch_download(s) for s in slots]
+        q = [c.batch_query(s) for s in slots]
+    for (g, W), (cp, t), qq in zip(gs, res, q):
+        ref, tref = alone(g, W, 37, precision)
+        e = rel_err(cons(cp), cons(ref), "inf")
+        print(g.nx, g.ny, g.nz, "max rel Linf", e.max(), "t", t, tref)
+        assert qq["steps"] == 37
+        assert t == pytest.approx(tref, rel=1e-14)
+        assert np.all(e <= 1e-12), e
+
+
+def test_admission_and_retirement_mid_run():
+    case("batch admit / retire mid-run")
+    gs = grids()
+    (gA, WA), (gB, WB), (gC, WC) = gs[0], gs[3], gs[1]
+    with solver.Context(solver.cartesian_config()) as c:
+        a = c.batch_add(gA, WA)
+        c.batch_step(10)
+        b = c.batch_add(gB, WB)
+        c.batch_step(10)
+        c.batch_remove(a)
+        cc = c.batch_add(gC, WC)
+        c.batch_step(5)
+        cpB, _ = c.batch_download(b)
+        cpC, _ = c.batch_download(cc)
+        assert c.batch_query(b)["steps"] == 15 and c.batch_query(cc)["steps"] == 5
+    for cp, (g, W), n in ((cpB, (gB, WB), 15), (cpC, (gC, WC), 5)):
+        ref, _ = alone(g, W, n)
+        e = rel_err(cons(cp), cons(ref), "inf")
+        assert np.all(e <= 1e-12), e
+
+
+def test_autotune_restores_state():
+    case("batch autotune restores state")
+    gs = grids()
+    with solver.Context(solver.cartesian_config()) as c:
+        slots = [c.batch_add(g, W) for g, W in gs]
+        c.batch_step(4)
+        tx, ty, bpc = c.batch_autotune(probe_steps=3)
+        assert (tx, ty) in {(32, 8), (16, 16), (64, 4), (16, 8)} and bpc in (1, 2, 4, 8, 64)
+        assert all(c.batch_query(s)["steps"] == 4 for s in slots)
+        c.batch_step(6)
+        res = [c.batch_download(s)[0] for s in slots]
+    for (g, W), cp in zip(gs, res):
+        ref, _ = alone(g, W, 10)
+        e = rel_err(cons(cp), cons(ref), "inf")
+        assert np.all(e <= 1e-12), e
+
+
+def test_batch_residual_matches_single():
+    case("batch residual vs single-grid residual")
+    g, W = grids()[2]
+    with solver.Context(solver.cartesian_config()) as c:
+        s = c.batch_add(g, W)
+        c.batch_step(9)
+        r_batch = c.batch_query(s)["residual"]
+    with solver.Context(solver.cartesian_config()) as c:
+        c.upload_structured(g, W)
+        c.dt_from_state()
+        c.step(8)
+        r_single = c.step_residual()
+    assert r_batch > 0 and r_batch == pytest.approx(r_single, rel=1e-9)
