@@ -63,10 +63,11 @@ struct setmhd_ctx {
   int* d_tgrid[2] = {nullptr, nullptr};
   int* d_tfirst[2] = {nullptr, nullptr};
   int* d_counts[2] = {nullptr, nullptr};
+  int* d_bfirst[2] = {nullptr, nullptr};
   BatchLaunch bl[2]{};
   bool tables_dirty = true;     // tables need re-writing (no new graph)
   std::vector<BatchGrid> stage_g[2];
-  std::vector<int> stage_t[2], stage_f[2];
+  std::vector<int> stage_t[2], stage_f[2], stage_b[2];
   int stage_n[2][2] = {{0, 0}, {0, 0}};
   std::vector<struct SlotScal> scal_host;  // host mirror after each batch step
   bool mirror_ok = false;
@@ -127,8 +128,10 @@ void free_batch(setmhd_ctx* c) {
   }
   c->slots.clear();
   for (int d = 0; d < 2; d++) {
-    for (void* p : {(void*)c->d_bgrid[d], (void*)c->d_tgrid[d], (void*)c->d_tfirst[d], (void*)c->d_counts[d]})
+    for (void* p : {(void*)c->d_bgrid[d], (void*)c->d_tgrid[d], (void*)c->d_tfirst[d], (void*)c->d_counts[d],
+                    (void*)c->d_bfirst[d]})
       if (p) hipFree(p);
+    c->d_bfirst[d] = nullptr;
     c->d_bgrid[d] = nullptr;
     c->d_tgrid[d] = nullptr;
     c->d_tfirst[d] = nullptr;

@@ -70,11 +70,15 @@ struct BatchGrid {
   int* err;
 };
 
+// largest persistent grid of a batch launch, in blocks per compute unit
+constexpr int kMaxBlocksPerCu = 64;
+
 struct BatchLaunch {
   const BatchGrid* grids;  // device array [cap_grids]
   const int* tile_grid;    // [cap_tiles] grid of each tile
   const int* tile_first;   // [cap_grids] first global tile of each grid
   const int* counts;       // device [2]: tiles, grids in use
+  const int* blk_first;    // [nblocks + 1] tile range of each block, split by cost
   int cap_tiles, cap_grids;
   int nblocks;             // persistent grid size
   int dim;                 // 2 or 3: all grids of one launch share it

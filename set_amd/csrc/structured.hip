@@ -13,8 +13,9 @@
 //     x- and y-faces of the plane are read from LDS, the z-face below each
 //     cell comes from the previous plane kept in registers;
 //   * face work is balanced: every thread forms one x-face and one y-face of
-//     the plane and one z-face of its column; the TX + TY faces on the far
-//     tile edges go to the first TX + TY lanes;
+//     the plane and one z-face of its column; the halo cells and the TX + TY
+//     faces on the far tile edges are dealt to different waves, rotated
+//     between odd and even tiles, so no wave carries two extra passes;
 //   * the CFL candidate is the per-axis max wave speed (min of c/s over faces
 //     = c/max s, division being monotone), reduced per wave by shuffles and
 //     per block through LDS, then one atomic per block;
@@ -212,13 +213,29 @@ __device__ __forceinline__ void finalize(const FusedArgs& a, double dt, unsigned
 // stage the planes, form the faces, update the cells, and fold the tile's
 // CFL candidate into the grid's next dt slot.  book: this tile also does the
 // step's bookkeeping (t += dt, re-arm the dt slot after next).
+// Block-level results of one tile, valid in thread 0: the tile's CFL
+// candidate for the next step and its density-residual partial sums.
+struct TileRed {
+  double dt, rd, rr;
+};
+
 template <class R, int TX, int TY, int DIM, int MODE>
-__device__ __forceinline__ void fused_tile(const FusedArgs& a, int b, bool book,
-                                           Smem<R, TX, TY>& sm, double* red) {
+__device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool book,
+                                              Smem<R, TX, TY>& sm, double* red) {
   constexpr int NH = 2 * TX + 2 * TY;
-  static_assert(TX <= 64 && 64 + TY <= TX * TY && NH <= TX * TY, "edge-face lane layout");
+  constexpr int NW = TX * TY / 64;  // waves per block
+  static_assert(TX <= 64 && TY <= 64 && NW >= 2 && NH <= (NW - 1) * 64 + 64, "extra-work layout");
   const SGrid& g = a.g;
   const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+  // Work beyond one cell per lane -- the halo ring's cells and the faces on
+  // the tile's far x and far y edges -- is spread over the waves by role:
+  // roles 0.. take 64 halo cells each, role NW-1 the far-x faces, role NW-2
+  // the far-y faces.  Odd tiles rotate the roles by NW/2 so that the two
+  // blocks sharing a CU load its SIMDs evenly.
+  const int lane = tid & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int role = (wid + ((b & 1) ? NW / 2 : 0)) % NW;
+  const int hid = role * 64 + lane;  // halo ring index of this lane
 
   const int ntx = (g.nx + TX - 1) / TX, nty = (g.ny + TY - 1) / TY;
   const int tile_x = b % ntx, rest = b / ntx, tile_y = rest % nty, kc = rest / nty;
@@ -240,12 +257,12 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int b, bool book,
 
   // halo cell of this thread (ring around the tile, corners excluded)
   int hly = 0, hlx = 0, hgi = 0, hgj = 0;
-  const bool has_h = tid < NH;
+  const bool has_h = hid < NH;
   if (has_h) {
-    if (tid < TX) { hlx = tid + 1; hly = 0; hgi = i0 + tid; hgj = j0 - 1; }
-    else if (tid < 2 * TX) { hlx = tid - TX + 1; hly = TY + 1; hgi = i0 + tid - TX; hgj = j0 + TY; }
-    else if (tid < 2 * TX + TY) { hlx = 0; hly = tid - 2 * TX + 1; hgi = i0 - 1; hgj = j0 + tid - 2 * TX; }
-    else { hlx = TX + 1; hly = tid - 2 * TX - TY + 1; hgi = i0 + TX; hgj = j0 + tid - 2 * TX - TY; }
+    if (hid < TX) { hlx = hid + 1; hly = 0; hgi = i0 + hid; hgj = j0 - 1; }
+    else if (hid < 2 * TX) { hlx = hid - TX + 1; hly = TY + 1; hgi = i0 + hid - TX; hgj = j0 + TY; }
+    else if (hid < 2 * TX + TY) { hlx = 0; hly = hid - 2 * TX + 1; hgi = i0 - 1; hgj = j0 + hid - 2 * TX; }
+    else { hlx = TX + 1; hly = hid - 2 * TX - TY + 1; hgi = i0 + TX; hgj = j0 + hid - 2 * TX - TY; }
     hgi = min(max(hgi, 0), g.nx - 1);
     hgj = min(max(hgj, 0), g.ny - 1);
   }
@@ -305,19 +322,20 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int b, bool book,
     __syncthreads();
 
     // x- and y-faces of the plane.  The TY faces on the tile's far x edge
-    // go to lanes 64..64+TY-1 (wave 1), the TX faces on the far y edge to
-    // lanes 0..TX-1 (wave 0), each as a second iteration of its loop.
+    // go to lanes 0..TY-1 of the role NW-1 wave, the TX faces on the far y
+    // edge to lanes 0..TX-1 of the role NW-2 wave, each as a second
+    // iteration of its loop.
     {
-      const int nx_it = (tid >= 64 && tid < 64 + TY) ? 2 : 1;
+      const int nx_it = (role == NW - 1 && lane < TY) ? 2 : 1;
       for (int r = 0; r < nx_it; r++) {
-        const int row = r == 0 ? ty + 1 : tid - 64 + 1;
+        const int row = r == 0 ? ty + 1 : lane + 1;
         const int col = r == 0 ? tx : TX;
         lds_face<0>(sm, row, col, row, col + 1, smax[0], ok);
       }
-      const int ny_it = tid < TX ? 2 : 1;
+      const int ny_it = (role == NW - 2 && lane < TX) ? 2 : 1;
       for (int r = 0; r < ny_it; r++) {
         const int row = r == 0 ? ty : TY;
-        const int col = r == 0 ? tx + 1 : tid + 1;
+        const int col = r == 0 ? tx + 1 : lane + 1;
         lds_face<1>(sm, row, col, row + 1, col, smax[1], ok);
       }
     }
@@ -396,60 +414,89 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int b, bool book,
   }
   if (!ok) atomicOr(a.err, kErrRegion);
 
-  const int lane = tid & 63, wid = tid >> 6;
-  if (MODE != kModeHeun2) {  // CFL minimum for the next step
-    double cand = g.cfl * g.distx / smax[0];
+  // Reductions over the block: CFL minimum for the next step and, when
+  // asked for, the density-residual sums (one LDS round, no atomics here).
+  const bool want_rs = a.resid != nullptr && MODE != kModeHeun1;
+  double cand = kDtReset;
+  if (MODE != kModeHeun2) {
+    cand = g.cfl * g.distx / smax[0];
     cand = fmin(cand, g.cfl * g.disty / smax[1]);
     if (DIM == 3) cand = fmin(cand, g.cfl * g.distz / smax[2]);
     cand = fmin(cand, g.dt_cap);
     cand = wave_min(cand);
-    if (lane == 0) red[wid] = cand;
-    __syncthreads();
-    if (tid == 0) {
-      double m = red[0];
-      for (int w = 1; w < TX * TY / 64; w++) m = fmin(m, red[w]);
-      atomic_min_pos(&a.dt.slot[(a.cur + 1) % 3], m);
-    }
   }
-  if (a.resid != nullptr && MODE != kModeHeun1) {
+  if (want_rs) {
     rs_d = wave_sum(rs_d);
     rs_r = wave_sum(rs_r);
-    if (lane == 0) {
-      atomicAdd(&a.resid[0], rs_d);
-      atomicAdd(&a.resid[1], rs_r);
+  }
+  if (lane == 0) {
+    red[wid] = cand;
+    red[NW + wid] = rs_d;
+    red[2 * NW + wid] = rs_r;
+  }
+  __syncthreads();
+  TileRed out{kDtReset, 0.0, 0.0};
+  if (tid == 0) {
+    out = TileRed{red[0], red[NW], red[2 * NW]};
+    for (int w = 1; w < NW; w++) {
+      out.dt = fmin(out.dt, red[w]);
+      out.rd += red[NW + w];
+      out.rr += red[2 * NW + w];
     }
   }
+  return out;
 }
 
 // One grid: one block per tile, tiles remapped so each XCD (blockIdx % 8)
 // gets a contiguous range (neighbouring tiles share halo lines in its L2).
 template <class R, int TX, int TY, int DIM, int MODE>
-__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))) void k_fused(FusedArgs a) {
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 2))) void k_fused(FusedArgs a) {
   __shared__ Smem<R, TX, TY> sm;
-  __shared__ double red[TX * TY / 64];
+  __shared__ double red[3 * TX * TY / 64];
   int b = blockIdx.x;
   const int nblk = gridDim.x;
   if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
-  fused_tile<R, TX, TY, DIM, MODE>(a, b, blockIdx.x == 0, sm, red);
+  const TileRed r = fused_tile<R, TX, TY, DIM, MODE>(a, b, blockIdx.x == 0, sm, red);
+  if (threadIdx.x == 0) {
+    if (MODE != kModeHeun2) atomic_min_pos(&a.dt.slot[(a.cur + 1) % 3], r.dt);
+    if (a.resid != nullptr && MODE != kModeHeun1) {
+      atomicAdd(&a.resid[0], r.rd);
+      atomicAdd(&a.resid[1], r.rr);
+    }
+  }
 }
 
 // Many grids, one persistent launch per step.  Block b walks a contiguous
-// range of the global tile list.  Each grid's step index lives on the
+// range of the global tile list, cut so that every range carries about the
+// same number of planes (ranges in block order: dealing them out per XCD
+// measured 25 % slower for this persistent launch).  Each grid's step index lives on the
 // device (BatchGrid::nstep): it selects the dt slot and the ping-pong
 // buffer, so the launch has no per-step arguments and many steps can be
 // captured in one graph.  k_batch_book, launched after it on the same
 // stream, closes the step of every grid.
 template <class R, int TX, int TY, int DIM>
-__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))) void k_fused_batch(
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 2))) void k_fused_batch(
     const BatchGrid* __restrict__ grids, const int* __restrict__ tile_grid,
-    const int* __restrict__ tile_first, const int* __restrict__ counts) {
+    const int* __restrict__ tile_first, const int* __restrict__ blk_first) {
   __shared__ Smem<R, TX, TY> sm;
-  __shared__ double red[TX * TY / 64];
-  const int ntiles = counts[0];  // on the device: the batch can change between graph replays
-  const int per = (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
-  const int t0 = (int)blockIdx.x * per, t1 = min(t0 + per, ntiles);
+  __shared__ double red[3 * TX * TY / 64];
+  // tables on the device: the batch can change between graph replays
+  const int t0 = blk_first[blockIdx.x], t1 = blk_first[blockIdx.x + 1];
+  // thread 0 folds the block's results over consecutive tiles of one grid
+  // and publishes them (one atomic each) when the grid changes
+  int cur = -1;
+  TileRed acc{kDtReset, 0.0, 0.0};
+  auto flush = [&]() {
+    const BatchGrid& fg = grids[cur];
+    atomic_min_pos(&fg.dt.slot[(*fg.nstep + 1) % 3], acc.dt);
+    atomicAdd(&fg.resid[0], acc.rd);
+    atomicAdd(&fg.resid[1], acc.rr);
+    acc = TileRed{kDtReset, 0.0, 0.0};
+  };
   for (int t = t0; t < t1; t++) {
     const int gi = tile_grid[t];
+    if (threadIdx.x == 0 && gi != cur && cur >= 0) flush();
+    cur = gi;
     const BatchGrid& bg = grids[gi];
     const int n = *bg.nstep;
     FusedArgs a;
@@ -462,9 +509,15 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))) voi
     a.kchunk = bg.kchunk;
     a.resid = bg.resid;
     a.err = bg.err;
-    fused_tile<R, TX, TY, DIM, kModeEuler>(a, t - tile_first[gi], false, sm, red);
+    const TileRed r = fused_tile<R, TX, TY, DIM, kModeEuler>(a, t - tile_first[gi], false, sm, red);
+    if (threadIdx.x == 0) {
+      acc.dt = fmin(acc.dt, r.dt);
+      acc.rd += r.rd;
+      acc.rr += r.rr;
+    }
     __syncthreads();  // LDS is reused by the next tile
   }
+  if (threadIdx.x == 0 && cur >= 0) flush();
 }
 
 // Closes one batch step for each grid: t += dt (cuf_kernel.cuf:792-794),
@@ -519,10 +572,10 @@ template <class R, int TX, int TY>
 static hipError_t launch_batch_tile(hipStream_t s, const BatchLaunch& L) {
   if (L.dim == 3)
     hipLaunchKernelGGL((k_fused_batch<R, TX, TY, 3>), dim3(L.nblocks), dim3(TX * TY), 0, s, L.grids,
-                       L.tile_grid, L.tile_first, L.counts);
+                       L.tile_grid, L.tile_first, L.blk_first);
   else
     hipLaunchKernelGGL((k_fused_batch<R, TX, TY, 2>), dim3(L.nblocks), dim3(TX * TY), 0, s, L.grids,
-                       L.tile_grid, L.tile_first, L.counts);
+                       L.tile_grid, L.tile_first, L.blk_first);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_batch_book, dim3((L.cap_grids + 63) / 64), dim3(64), 0, s, L.grids, L.counts);

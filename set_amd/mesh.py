@@ -231,7 +231,11 @@ class Wind:
 
 
 def heliosphere_state(g: Grid, inflow: Inflow = Inflow(), wind: Wind = Wind()):
-    """Cell_par (Nc, 16): rho,u,v,w,p,Bx,By,Bz then 8 neutral moments (0)."""
+    """Cell_par (Nc, 16): rho,u,v,w,p,Bx,By,Bz then 8 neutral moments (0).
+
+    Wind cells (|centre| < r_in) get rho = 1/r^2 and u = speed * centre/r
+    with r floored at half the smallest cell size, so a cell centred on
+    the origin (odd cell counts) has finite density and zero velocity."""
     c = g.cell_centers()
     nc = len(c)
     W = np.zeros((nc, NPAR))
@@ -245,7 +249,8 @@ def heliosphere_state(g: Grid, inflow: Inflow = Inflow(), wind: Wind = Wind()):
     W[:, 6] = inflow.b * np.sin(th)
     r = np.sqrt(c[:, 0] ** 2 + c[:, 1] ** 2 + c[:, 2] ** 2)
     inner = r < wind.r_in
-    rr = r[inner]
+    h = 0.5 * min(g.dx, g.dy, g.dz if g.dim == 3 else g.dx)
+    rr = np.maximum(r[inner], h)
     W[inner, 0] = 1.0 / rr ** 2
     W[inner, 1:4] = wind.speed * c[inner] / rr[:, None]
     W[inner, 4] = wind.p_over_rho * W[inner, 0]
