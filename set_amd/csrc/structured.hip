@@ -213,15 +213,58 @@ __device__ __forceinline__ void finalize(const FusedArgs& a, double dt, unsigned
 // stage the planes, form the faces, update the cells, and fold the tile's
 // CFL candidate into the grid's next dt slot.  book: this tile also does the
 // step's bookkeeping (t += dt, re-arm the dt slot after next).
+// Raw FP64 state of the own cells of one plane of a tile, [field][row][x],
+// filled by LDS DMA (global_load_lds).
+template <int TX, int TY>
+struct Stage {
+  double own[8][TY][TX];
+};
+
+typedef const void __attribute__((address_space(1)))* gptr_t;
+typedef void __attribute__((address_space(3)))* lptr_t;
+
+// Offset (in doubles, within one plane of one field) of the two cells a
+// lane moves for chunk c of the tile's row-major cell list: 128 cells per
+// wave instruction, 16 bytes (two x-neighbours) per lane.  x is clamped to
+// nx - 2 (nx even: pairs never straddle a row end), y to ny - 1, so every
+// address is inside the state and every in-grid cell is staged exactly.
+template <int TX>
+__device__ __forceinline__ unsigned dma_lane_offset(const SGrid& g, int i0, int j0, int c, int lane) {
+  const int e = c * 128 + 2 * lane;
+  return (unsigned)min(j0 + e / TX, g.ny - 1) * (unsigned)g.nx + (unsigned)min(i0 + e % TX, g.nx - 2);
+}
+
+// Issues the DMA of plane k's own cells.  Instruction q moves chunk
+// q % NCH of field q / NCH; wave w issues q = w, w + NW, ...  With NW a
+// multiple of NCH every instruction of a wave uses the same chunk, whose
+// lane offset is off.  Lane l lands at the instruction's LDS base + 16 l.
+template <int TX, int TY>
+__device__ __forceinline__ void dma_plane(Stage<TX, TY>& st, const FusedArgs& a, int k, int wid,
+                                          unsigned off) {
+  constexpr int NW = TX * TY / 64, NCH = TX * TY / 128;
+  static_assert(NW % NCH == 0, "one chunk per wave");
+  const double* base = a.Uf + (size_t)a.g.nx * a.g.ny * k + off;
+#pragma unroll
+  for (int q = 0; q < 8 * NCH / NW; q++) {
+    const int qq = wid + q * NW, v = qq / NCH, c = qq % NCH;
+    __builtin_amdgcn_global_load_lds((gptr_t)(base + v * a.g.nc),
+                                     (lptr_t)(&st.own[0][0][0] + v * TX * TY + c * 128), 16, 0, 0);
+  }
+}
+
+// vmcnt(0) only (expcnt and lgkmcnt left at their maxima)
+__device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Block-level results of one tile, valid in thread 0: the tile's CFL
 // candidate for the next step and its density-residual partial sums.
 struct TileRed {
   double dt, rd, rr;
 };
 
-template <class R, int TX, int TY, int DIM, int MODE>
+template <class R, int TX, int TY, int DIM, int MODE, bool DMA = false>
 __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool book,
-                                              Smem<R, TX, TY>& sm, double* red) {
+                                              Smem<R, TX, TY>& sm, double* red,
+                                              Stage<TX, TY>* st = nullptr) {
   constexpr int NH = 2 * TX + 2 * TY;
   constexpr int NW = TX * TY / 64;  // waves per block
   static_assert(TX <= 64 && TY <= 64 && NW >= 2 && NH <= (NW - 1) * 64 + 64, "extra-work layout");
@@ -285,8 +328,20 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
   // Software pipeline (FP64): the next plane's own and halo cells are loaded
   // while the current plane's faces are formed.  The mixed form skips it:
   // its 32 extra VGPRs would cost a wave per SIMD.
-  constexpr bool PF = sizeof(R) == 8;
+  // With DMA (3-D, even nx) the next plane goes global -> LDS directly
+  // while the faces are formed, using no registers.
+  static_assert(!DMA || DIM == 3, "plane DMA is for the z-march");
+  constexpr bool PF = sizeof(R) == 8 && !DMA;
   const unsigned hcol = (unsigned)hgi + (unsigned)g.nx * (unsigned)hgj;
+  // staged positions of this lane's own and halo cells (clamped as above)
+  const int sox = ic - i0, soy = jc - j0;
+  unsigned dma_off = 0;
+  if constexpr (DMA) {
+    dma_off = dma_lane_offset<TX>(g, i0, j0, wid % (TX * TY / 128), lane);
+    dma_plane<TX, TY>(*st, a, k0, wid, dma_off);
+    wait_vm();
+    __syncthreads();
+  }
   double Unx[8], Hnx[8];
   if constexpr (PF) {
     load8(a.Uf, g.nc, col0 + plane * k0, Unx);
@@ -307,6 +362,10 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
         load8(a.Uf, g.nc, col0 + plane * kn, Unx);
         if (has_h && k + 1 < k1) load8(a.Uf, g.nc, hcol + plane * kn, Hnx);
       }
+    } else if constexpr (DMA) {
+      if (has_h) load8(a.Uf, g.nc, hcol + plane * k, Uh);
+#pragma unroll
+      for (int v = 0; v < 8; v++) Uc[v] = st->own[v][soy][sox];
     } else {
       load8(a.Uf, g.nc, idx, Uc);
       if (has_h) load8(a.Uf, g.nc, hcol + plane * k, Uh);
@@ -320,6 +379,9 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
       put_q(sm, hly, hlx, qh);
     }
     __syncthreads();
+    // every wave has read the staged plane: stage plane k+1 (the epilogue's
+    // plane when k+1 == k1)
+    if constexpr (DMA) dma_plane<TX, TY>(*st, a, min(k + 1, g.nz - 1), wid, dma_off);
 
     // x- and y-faces of the plane.  The TY faces on the tile's far x edge
     // go to lanes 0..TY-1 of the role NW-1 wave, the TX faces on the far y
@@ -348,6 +410,7 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
       ok &= hlld(zp, zc, Fz, s);
       smax[2] = fmax(smax[2], (double)s);
     }
+    if constexpr (DMA) wait_vm();  // this wave's share of plane k+1 has landed
     __syncthreads();
 
     // signed face sum in the face-slot order (x-, x+, y-, y+, z-, z+)
@@ -388,7 +451,12 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
   }
 
   if (DIM == 3) {  // z-face above the chunk's last plane
-    if constexpr (!PF) load8(a.Uf, g.nc, col0 + plane * min(k1, g.nz - 1), Unx);
+    if constexpr (DMA) {
+#pragma unroll
+      for (int v = 0; v < 8; v++) Unx[v] = st->own[v][soy][sox];
+    } else if constexpr (!PF) {
+      load8(a.Uf, g.nc, col0 + plane * min(k1, g.nz - 1), Unx);
+    }
     CellQ<R> q;
     derive<R, true>(Unx, q);  // plane min(k1, nz-1)
     const Side<R> zc = side_of(q, 2);
@@ -449,14 +517,20 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
 
 // One grid: one block per tile, tiles remapped so each XCD (blockIdx % 8)
 // gets a contiguous range (neighbouring tiles share halo lines in its L2).
-template <class R, int TX, int TY, int DIM, int MODE>
+template <class R, int TX, int TY, int DIM, int MODE, bool DMA>
 __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 2))) void k_fused(FusedArgs a) {
   __shared__ Smem<R, TX, TY> sm;
   __shared__ double red[3 * TX * TY / 64];
   int b = blockIdx.x;
   const int nblk = gridDim.x;
   if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
-  const TileRed r = fused_tile<R, TX, TY, DIM, MODE>(a, b, blockIdx.x == 0, sm, red);
+  TileRed r;
+  if constexpr (DMA) {
+    __shared__ Stage<TX, TY> st;
+    r = fused_tile<R, TX, TY, DIM, MODE, true>(a, b, blockIdx.x == 0, sm, red, &st);
+  } else {
+    r = fused_tile<R, TX, TY, DIM, MODE>(a, b, blockIdx.x == 0, sm, red);
+  }
   if (threadIdx.x == 0) {
     if (MODE != kModeHeun2) atomic_min_pos(&a.dt.slot[(a.cur + 1) % 3], r.dt);
     if (a.resid != nullptr && MODE != kModeHeun1) {
@@ -542,7 +616,13 @@ static hipError_t launch_one(hipStream_t s, const FusedArgs& a) {
   const int ntx = (a.g.nx + TX - 1) / TX, nty = (a.g.ny + TY - 1) / TY;
   const int nkz = DIM == 3 ? (a.g.nz + a.kchunk - 1) / a.kchunk : 1;
   const long long nb = (long long)ntx * nty * nkz;
-  hipLaunchKernelGGL((k_fused<R, TX, TY, DIM, MODE>), dim3((unsigned)nb), dim3(TX * TY), 0, s, a);
+  // plane DMA: mixed form, 3-D, 32x8 tiles (the staging fits 3 blocks/CU
+  // beside the face data), rows of even length (16-byte pairs)
+  constexpr bool kDma = sizeof(R) == 4 && DIM == 3 && TX == 32 && TY == 8;
+  if (kDma && a.g.nx % 2 == 0)
+    hipLaunchKernelGGL((k_fused<R, TX, TY, DIM, MODE, kDma>), dim3((unsigned)nb), dim3(TX * TY), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_fused<R, TX, TY, DIM, MODE, false>), dim3((unsigned)nb), dim3(TX * TY), 0, s, a);
   return hipGetLastError();
 }
 

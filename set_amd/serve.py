@@ -46,13 +46,17 @@ def _latency_stats(done, wall):
 
 class Server:
     def __init__(self, precision=solver.PREC_FP64, check_every=50, max_steps=5000, tol=1e-6,
-                 reserve_bytes=4 << 30, max_active=512, autotune=True):
+                 reserve_bytes=4 << 30, max_active=512, max_active_cells=None, autotune=True):
         self.precision = precision
         self.check_every = check_every
         self.max_steps = max_steps
         self.tol = tol
         self.reserve = reserve_bytes
         self.max_active = max_active
+        # optional cap on the cells stepping at once (beyond what fills the
+        # GPU, more concurrency only stretches every request's latency);
+        # a request larger than the cap still runs, alone
+        self.max_active_cells = max_active_cells
         self.autotune = autotune
 
     def run(self, requests, states, clock=time.perf_counter, sleep=time.sleep, keep_results=False):
@@ -64,6 +68,7 @@ class Server:
         done = []
         tuned = not self.autotune
         spent = collections.Counter()  # seconds per server activity
+        active_cells = 0
         with solver.Context(solver.cartesian_config(precision=self.precision)) as ctx:
             t0 = clock()
             while pending or active:
@@ -75,11 +80,15 @@ class Server:
                     need = requests[i].cells * BYTES_PER_CELL
                     if free - need < self.reserve:
                         break  # admitted when memory frees up
+                    if (self.max_active_cells is not None and active
+                            and active_cells + requests[i].cells > self.max_active_cells):
+                        break  # admitted when running requests retire
                     pending.popleft()
                     ta = clock()
                     slot = ctx.batch_add(requests[i].grid, states[i])
                     spent["admit"] += clock() - ta
                     active[slot] = [i, 0, clock() - t0]
+                    active_cells += requests[i].cells
                     free -= need
                 if not active:
                     sleep(max(0.0, requests[pending[0]].arrival - (clock() - t0)))
@@ -105,6 +114,7 @@ class Server:
                         ctx.batch_remove(slot)
                         spent["retire"] += clock() - ta
                         del active[slot]
+                        active_cells -= requests[i].cells
                         done.append({"rid": requests[i].rid, "arrival": requests[i].arrival,
                                      "admitted": admitted, "done": clock() - t0,
                                      "steps": steps, "residual": q["residual"],

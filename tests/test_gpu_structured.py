@@ -122,6 +122,22 @@ def test_mixed_precision_error():
     assert e[0] <= 1e-5, e
 
 
+@pytest.mark.parametrize("nx", [66, 65])
+def test_mixed_plane_dma_matches_register_path(nx):
+    """The mixed 3-D kernel with 32x8 tiles stages each plane through LDS
+    DMA when nx is even (65: the register path).  Ragged tiles in x and y;
+    compared with the 16x16 kernel, which always loads through registers."""
+    case(f"structured 3-D {nx}x36x40 mixed, 32x8 vs 16x16, steps=5")
+    g = mesh.grid_3d(nx, 36, 40)
+    W = mesh.heliosphere_state(g, wind=mesh.Wind(r_in=0.3))
+    a, ta, _ = gpu_run(g, W, 5, precision=1, tile=(32, 8))
+    b, tb, _ = gpu_run(g, W, 5, precision=1, tile=(16, 16))
+    e = rel_err(cons(a), cons(b), "inf")
+    print("rel Linf per var", e, "bitwise", np.array_equal(a, b))
+    assert ta == tb
+    assert np.all(e <= 1e-9), e
+
+
 def test_free_stream_256cube_exact():
     """Full size: a uniform state is a fixed point, bit for bit (every face
     sees the same two states, so every face sum cancels exactly)."""
