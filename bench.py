@@ -48,7 +48,10 @@ def run_form(g, W, precision, time_scheme, steps, warmup):
     cfg = solver.cartesian_config(cfl=0.4, r_hold=0.1, precision=precision,
                                   time_scheme=time_scheme)
     with solver.Context(cfg) as c:
+        t0 = time.perf_counter()
         c.upload_structured(g, W)
+        c.sync()
+        t_up = time.perf_counter() - t0
         c.dt_from_state()
         if warmup:
             c.step(warmup)
@@ -59,6 +62,9 @@ def run_form(g, W, precision, time_scheme, steps, warmup):
         wall = time.perf_counter() - t0
         ms_ev = c.step_timed(steps)  # kernel time on the library's stream
         t_sim, dt = c.sync()
+        t0 = time.perf_counter()
+        c.download()
+        t_down = time.perf_counter() - t0
     launches = steps * (2 if time_scheme == 1 else 1)
     bytes_per_cell = 320 if time_scheme == 1 else 128
     return {
@@ -66,6 +72,11 @@ def run_form(g, W, precision, time_scheme, steps, warmup):
         "ms_per_step": 1e3 * wall / steps,
         "kernel_ms_per_launch": ms_ev / launches,
         "achieved_GBs": g.n_cell * bytes_per_cell * steps / (ms_ev * 1e-3) / 1e9,
+        # the same steps with one upload and one download of Cell_par
+        # (host buffers over PCIe, conversion kernels included)
+        "upload_s": t_up,
+        "download_s": t_down,
+        "pcie_inclusive_value": g.n_cell * steps / (wall + t_up + t_down),
         "t_sim": t_sim,
         "dt": dt,
     }

@@ -46,7 +46,8 @@ def _latency_stats(done, wall):
 
 class Server:
     def __init__(self, precision=solver.PREC_FP64, check_every=50, max_steps=5000, tol=1e-6,
-                 reserve_bytes=4 << 30, max_active=512, max_active_cells=None, autotune=True):
+                 reserve_bytes=4 << 30, max_active=512, max_active_cells=None, autotune=True,
+                 context=None):
         self.precision = precision
         self.check_every = check_every
         self.max_steps = max_steps
@@ -58,6 +59,8 @@ class Server:
         # a request larger than the cap still runs, alone
         self.max_active_cells = max_active_cells
         self.autotune = autotune
+        # context factory (a solver.Context by default; tests pass a model)
+        self.context = context or (lambda: solver.Context(solver.cartesian_config(precision=precision)))
 
     def run(self, requests, states, clock=time.perf_counter, sleep=time.sleep, keep_results=False):
         """requests: workloads.Request list; states: their initial Cell_par
@@ -69,7 +72,7 @@ class Server:
         tuned = not self.autotune
         spent = collections.Counter()  # seconds per server activity
         active_cells = 0
-        with solver.Context(solver.cartesian_config(precision=self.precision)) as ctx:
+        with self.context() as ctx:
             t0 = clock()
             while pending or active:
                 now = clock() - t0

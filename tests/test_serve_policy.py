@@ -1,0 +1,140 @@
+"""Server policy on the CPU: admission by free device memory in arrival
+order, retirement on the residual test or the step cap, the active-cell
+cap, and the latency statistics.  The batch context is replaced by a model
+whose clock advances with the work it is given, so these tests run without
+a GPU (the GPU behaviour of the same loop is in test_gpu_serve.py)."""
+import numpy as np
+import pytest
+
+from set_amd import mesh, serve, workloads
+
+
+class SimClock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        return self.t
+
+    def sleep(self, dt):
+        self.t += max(0.0, dt)
+
+
+class ModelContext:
+    """setmhd_batch_* model: a request of n cells converges after conv[n]
+    steps; stepping costs cells / rate seconds of the shared clock."""
+
+    def __init__(self, clock, free_bytes, conv, rate=1e9):
+        self.clock, self.free, self.conv, self.rate = clock, free_bytes, conv, rate
+        self.slots = {}
+        self.log = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        return False
+
+    def mem_info(self):
+        return self.free, 1 << 40
+
+    def batch_add(self, grid, state):
+        s = max(self.slots, default=-1) + 1
+        self.slots[s] = [grid.n_cell, 0]
+        self.free -= grid.n_cell * serve.BYTES_PER_CELL
+        self.log.append(("add", grid.n_cell, self.clock()))
+        return s
+
+    def batch_remove(self, s):
+        n, _ = self.slots.pop(s)
+        self.free += n * serve.BYTES_PER_CELL
+        self.log.append(("remove", n, self.clock()))
+
+    def batch_step(self, n):
+        cells = sum(c for c, _ in self.slots.values())
+        self.clock.t += n * cells / self.rate
+        for v in self.slots.values():
+            v[1] += n
+
+    def batch_query(self, s):
+        n, steps = self.slots[s]
+        return {"steps": steps, "t": 0.0, "dt": 1e-4,
+                "residual": 1e-7 if steps >= self.conv[n] else 1e-3}
+
+    def batch_download(self, s):
+        return np.zeros((self.slots[s][0], 16)), 0.0
+
+    def batch_autotune(self, probe_steps=2):
+        return (32, 8, 2)
+
+
+def requests(sizes, arrivals):
+    out = []
+    for i, (n, t) in enumerate(zip(sizes, arrivals)):
+        g = mesh.grid_2d(n, 1)
+        out.append(workloads.Request(i, t, g, mesh.Inflow()))
+    return out
+
+
+def run(reqs, free, conv, **kw):
+    clock = SimClock()
+    ctx = ModelContext(clock, free, conv)
+    srv = serve.Server(check_every=50, max_steps=kw.pop("max_steps", 5000), autotune=False,
+                       reserve_bytes=0, context=lambda: ctx, **kw)
+    states = [None] * len(reqs)
+    done, stats = srv.run(reqs, states, clock=clock, sleep=clock.sleep)
+    return done, stats, ctx
+
+
+def test_admission_waits_for_free_memory():
+    reqs = requests([1000, 2000, 3000], [0.0, 0.0, 0.0])
+    conv = {1000: 100, 2000: 200, 3000: 300}
+    # room for the first two together; the third fits only once both left
+    done, stats, ctx = run(reqs, 3500 * serve.BYTES_PER_CELL, conv)
+    adds = [e for e in ctx.log if e[0] == "add"]
+    removes = [e for e in ctx.log if e[0] == "remove"]
+    assert [a[1] for a in adds] == [1000, 2000, 3000]  # arrival order
+    assert adds[0][2] == adds[1][2] == 0.0
+    assert [r[1] for r in removes[:2]] == [1000, 2000]
+    assert adds[2][2] >= removes[1][2] > removes[0][2]
+    assert stats["solves"] == 3 and stats["converged"] == 3
+
+
+def test_all_admitted_when_memory_allows_and_retired_on_convergence():
+    reqs = requests([1000, 2000, 3000], [0.0, 0.0, 0.0])
+    conv = {1000: 120, 2000: 260, 3000: 40}
+    done, stats, ctx = run(reqs, 1 << 40, conv)
+    by = {r["cells"]: r for r in done}
+    for n, c in conv.items():
+        # retired at the first check (every 50 steps) at or after convergence
+        assert by[n]["steps"] == 50 * int(np.ceil(c / 50))
+        assert by[n]["converged"]
+    assert [r["cells"] for r in done] == [3000, 1000, 2000]
+
+
+def test_step_cap_retires_unconverged():
+    reqs = requests([1000], [0.0])
+    done, stats, _ = run(reqs, 1 << 40, {1000: 10 ** 9}, max_steps=300)
+    assert done[0]["steps"] == 300 and not done[0]["converged"]
+    assert stats["converged"] == 0
+
+
+def test_active_cell_cap_queues_in_arrival_order():
+    reqs = requests([1000, 1000, 1000], [0.0, 0.0, 0.0])
+    conv = {1000: 50}
+    done, stats, ctx = run(reqs, 1 << 40, conv, max_active_cells=2000)
+    live, peak = 0, 0
+    for e in ctx.log:
+        live += e[1] if e[0] == "add" else -e[1]
+        peak = max(peak, live)
+    assert peak == 2000 and stats["solves"] == 3
+
+
+def test_late_arrival_waits_and_latency_stats():
+    reqs = requests([1000, 1000], [0.0, 5.0])
+    done, stats, _ = run(reqs, 1 << 40, {1000: 100})
+    lat = sorted(r["done"] - r["arrival"] for r in done)
+    assert done[1]["admitted"] >= 5.0
+    assert stats["latency_p50_s"] == pytest.approx(np.percentile(lat, 50))
+    assert stats["latency_p99_s"] == pytest.approx(np.percentile(lat, 99))
+    assert stats["solves_per_hour"] == pytest.approx(3600 * 2 / stats["wall_s"])
