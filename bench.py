@@ -30,15 +30,18 @@ PEAK_HBM_GBS = 8000.0
 METRIC = "HLLD cell-updates/s FP64 vs mixed (% HBM peak); p50/p99 solve latency"
 
 
-def traffic_from_profiles(name):
-    """HBM bytes per launch measured by rocprofv3 PMC runs, when a summary
-    for this kernel/workload is committed (profiles/<name>.txt holds a line
-    'traffic_bytes_per_launch: <int>')."""
+PEAK_FP64_TFLOPS = 78.6
+
+
+def from_profiles(name, key):
+    """A figure measured by rocprofv3 PMC runs, when a summary for this
+    kernel/workload is committed (profiles/<name>.txt holds 'key: value'),
+    e.g. traffic_bytes_per_launch or fp64_flops_per_cell_update."""
     p = os.path.join(ROOT, "profiles", name + ".txt")
     if not os.path.exists(p):
         return None
     for line in open(p):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return int(float(line.split(":")[1]))
     return None
 
@@ -125,7 +128,18 @@ def serving_form(n_requests, max_steps, precision=0):
             "solves_per_hour_ratio": b["solves_per_hour"] / q["solves_per_hour"]}
 
 
-def cpu_baseline(n, steps, threads):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_rate(n, steps, threads, runs=5):
+    """Median over runs (after a warm-up step) of the oracle's cell-updates/s."""
     import oracle
     from set_amd import mesh
     g = mesh.grid_3d(n, n, n)
@@ -134,15 +148,24 @@ def cpu_baseline(n, steps, threads):
     s = oracle.Solver(fm, W, oracle.cartesian_config(), nthreads=threads)
     s.dt_next = s.dt_scan()
     s.run(1)  # warm-up
-    t0 = time.perf_counter()
-    st = s.run(steps)
-    wall = time.perf_counter() - t0
-    assert st == 0
-    return {"value": g.n_cell * steps / wall, "unit": "cell-updates/s", "cores": threads,
-            "kind": "port",
+    rates = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        st = s.run(steps)
+        rates.append(g.n_cell * steps / (time.perf_counter() - t0))
+        assert st == 0
+    return sorted(rates)[len(rates) // 2]
+
+
+def cpu_baseline(n, steps, threads):
+    v = cpu_rate(n, steps, threads)
+    v1 = cpu_rate(max(32, n // 2), max(1, steps // 2), 1)
+    return {"value": v, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "single_core_value": v1, "cpu_model": cpu_model(), "nproc": os.cpu_count(),
             "sample": f"oracle (C restatement of chlld + face/cell passes) on the same "
-                      f"heliosphere set-up at {n}^3, {steps} forward-Euler steps, "
-                      f"{threads} OpenMP threads"}
+                      f"heliosphere set-up at {n}^3, median of 5 runs of {steps} forward-Euler "
+                      f"steps on {threads} OpenMP threads; single core: {max(32, n // 2)}^3, "
+                      f"{max(1, steps // 2)} steps"}
 
 
 def main():
@@ -154,7 +177,7 @@ def main():
     ap.add_argument("--forms", default="fp64,mixed,heun,batch64,serving")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-max-steps", type=int, default=5000)
@@ -201,10 +224,16 @@ def main():
             except Exception as e:  # a failing extra form must not lose the headline line
                 forms[name] = {"error": repr(e)}
 
-    traffic = traffic_from_profiles(f"fused_fp64_euler_{n}cube")
+    traffic = from_profiles(f"fused_fp64_euler_{n}cube", "traffic_bytes_per_launch")
     achieved = fp["achieved_GBs"]
     roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": achieved / PEAK_HBM_GBS, "traffic": traffic}
+    # FP64 flops of the same kernel (VALU counters of the committed profile)
+    fl = from_profiles(f"fused_fp64_euler_{n}cube", "fp64_flops_per_cell_update")
+    if fl:
+        tf = fl * g.n_cell / (fp["kernel_ms_per_launch"] * 1e-3) / 1e12
+        fp["compute"] = {"flops_per_cell_update": fl, "achieved": tf, "peak": PEAK_FP64_TFLOPS,
+                         "unit": "TFLOP/s (FP64 vector)", "frac": tf / PEAK_FP64_TFLOPS}
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

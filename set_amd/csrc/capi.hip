@@ -537,8 +537,8 @@ int setmhd_upload_structured(setmhd_ctx* c, int32_t nx, int32_t ny, int32_t nz, 
   if ((st = dalloc(c, &c->d_U, (size_t)nc * 8)) || (st = dalloc(c, &c->d_U2, (size_t)nc * 8)) ||
       (st = dalloc(c, &c->d_H16, (size_t)nc * 8)))
     return st;
-  // stage Cell_par through the second state buffer's memory when it is big
-  // enough (8*nc doubles < 16*nc): use a temporary instead.
+  // Cell_par (16 doubles per cell) is staged in a temporary device buffer
+  // and converted there (conserved state, neutral moments).
   double* stage = nullptr;
   if ((st = dalloc(c, &stage, (size_t)nc * 16))) return st;
   HIPCHK(c, hipMemcpyAsync(stage, cell_par, (size_t)nc * 16 * 8, hipMemcpyHostToDevice, c->stream));
