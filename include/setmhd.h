@@ -175,11 +175,29 @@ int setmhd_batch_add(setmhd_ctx *ctx, int32_t nx, int32_t ny, int32_t nz,
                      const double *box /*[6]*/, const double *cell_par,
                      double t0, int32_t *slot);
 
-/* Retires a grid and frees its device memory. */
+/* Retires a grid: it takes part in no further step.  Its state stays
+ * readable (setmhd_batch_query / setmhd_batch_download) until the next
+ * setmhd_batch_wait or setmhd_batch_step returns; its memory is freed
+ * there. */
 int setmhd_batch_remove(setmhd_ctx *ctx, int32_t slot);
 
-/* Advances every admitted grid by nsteps steps. */
+/* Advances every admitted grid by nsteps steps and waits for them. */
 int setmhd_batch_step(setmhd_ctx *ctx, int32_t nsteps);
+
+/* Queues nsteps steps of the admitted grids and returns at once.  While
+ * they run, setmhd_batch_add (the new grid joins the next step) and
+ * setmhd_batch_download / _query of retired or newly admitted grids proceed
+ * on a second stream; calls that need a grid of the running step wait for
+ * it. */
+int setmhd_batch_step_async(setmhd_ctx *ctx, int32_t nsteps);
+
+/* *done = 1 when the queued steps have finished (or none are queued),
+ * else 0; never blocks. */
+int setmhd_batch_poll(setmhd_ctx *ctx, int32_t *done);
+
+/* Completes the queued steps: per-grid bookkeeping becomes current, device
+ * errors are reported, retired grids are freed. */
+int setmhd_batch_wait(setmhd_ctx *ctx);
 
 /* Steps since admission, time, next dt and ||rho^{n+1}-rho^n||_1/||rho^n||_1
  * of the slot's last step (any pointer may be NULL). */

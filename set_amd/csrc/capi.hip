@@ -49,7 +49,10 @@ struct setmhd_ctx {
   int kchunk = 32;  // z-planes per block of the single structured grid
   // batch of structured grids (kind 3)
   struct Slot {
-    bool used = false;
+    bool used = false;     // in the batch
+    bool retired = false;  // removed, state kept readable until the next wait
+    bool fresh = false;    // admitted since the last completed step (no mirror entry)
+    bool in_step = false;  // part of the step in flight
     SGrid g{};
     double* buf[2] = {nullptr, nullptr};
     double* h16 = nullptr;
@@ -71,6 +74,16 @@ struct setmhd_ctx {
   int stage_n[2][2] = {{0, 0}, {0, 0}};
   std::vector<struct SlotScal> scal_host;  // host mirror after each batch step
   bool mirror_ok = false;
+  // overlap of admissions and downloads with a running batch step: a side
+  // stream for their copies and conversions, pinned memory for the step's
+  // mirror copy, grow-only device staging (no hipFree while a step runs)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_side = nullptr, ev_step = nullptr;
+  bool in_flight = false;
+  struct SlotScal* h_mirror = nullptr;  // pinned, slot_cap entries
+  double* up_stage = nullptr;
+  double* dn_stage = nullptr;
+  size_t up_cap = 0, dn_cap = 0;  // doubles
   hipGraph_t graph1 = nullptr, graphN = nullptr;
   hipGraphExec_t exec1 = nullptr, execN = nullptr;
   int graph_steps = 8;
@@ -121,6 +134,8 @@ void free_graphs(setmhd_ctx* c) {
 }
 
 void free_batch(setmhd_ctx* c) {
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->side) hipStreamSynchronize(c->side);
   free_graphs(c);
   for (auto& sl : c->slots) {
     for (double* p : {sl.buf[0], sl.buf[1], sl.h16})
@@ -141,6 +156,13 @@ void free_batch(setmhd_ctx* c) {
   if (c->d_slot_scal) hipFree(c->d_slot_scal);
   c->d_slot_scal = nullptr;
   c->slot_cap = 0;
+  if (c->h_mirror) hipHostFree(c->h_mirror);
+  c->h_mirror = nullptr;
+  for (double* p : {c->up_stage, c->dn_stage})
+    if (p) hipFree(p);
+  c->up_stage = c->dn_stage = nullptr;
+  c->up_cap = c->dn_cap = 0;
+  c->in_flight = false;
   c->dirty = true;
   c->tables_dirty = true;
   c->mirror_ok = false;
@@ -425,6 +447,9 @@ void setmhd_destroy(setmhd_ctx* c) {
   if (c->d_err) hipFree(c->d_err);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->ev_side) hipEventDestroy(c->ev_side);
+  if (c->ev_step) hipEventDestroy(c->ev_step);
+  if (c->side) hipStreamDestroy(c->side);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }

@@ -5,14 +5,14 @@ grid with its own inflow and field.  The server keeps every admitted
 request's state resident on the GPU as one slot of a batch context
 (setmhd_batch_*): all active requests advance together, one persistent
 fused launch per step per dimensionality, replayed from a captured graph.
-Between chunks of steps it
+Around chunks of steps it
 
   * admits arrived requests in arrival order while free HBM (hipMemGetInfo)
-    covers their state plus a reserve,
+    covers their state plus a reserve (uploads overlap the running chunk),
   * checks each request's relative density residual
     ||rho^{n+1} - rho^n||_1 / ||rho^n||_1 of its last step, and
   * retires a request when the residual is <= tol or it reached max_steps,
-    downloading its state as the result.
+    downloading its state as the result while the next chunk runs.
 
 A request that arrives while others are on the GPU therefore starts at the
 next chunk boundary instead of waiting for the running ones to finish.
@@ -47,7 +47,7 @@ def _latency_stats(done, wall):
 class Server:
     def __init__(self, precision=solver.PREC_FP64, check_every=50, max_steps=5000, tol=1e-6,
                  reserve_bytes=4 << 30, max_active=512, max_active_cells=None, autotune=True,
-                 context=None):
+                 context=None, poll_s=0.001):
         self.precision = precision
         self.check_every = check_every
         self.max_steps = max_steps
@@ -59,22 +59,29 @@ class Server:
         # a request larger than the cap still runs, alone
         self.max_active_cells = max_active_cells
         self.autotune = autotune
+        self.poll_s = poll_s  # host sleep while a chunk runs and requests are due
         # context factory (a solver.Context by default; tests pass a model)
         self.context = context or (lambda: solver.Context(solver.cartesian_config(precision=precision)))
 
     def run(self, requests, states, clock=time.perf_counter, sleep=time.sleep, keep_results=False):
         """requests: workloads.Request list; states: their initial Cell_par
-        (same order).  Returns per-request records and summary stats."""
+        (same order).  Returns per-request records and summary stats.
+
+        Each pass of the loop admits arrived requests (their uploads run on
+        the library's side stream, beside the chunk in flight), waits for
+        the chunk, retires converged or capped requests, starts the next
+        chunk, and downloads the retired results while it runs."""
         order = sorted(range(len(requests)), key=lambda i: requests[i].arrival)
         pending = collections.deque(order)
-        active = {}  # slot -> [request index, steps]
+        active = {}  # slot -> [request index, admitted time, steps done]
         done = []
         tuned = not self.autotune
         spent = collections.Counter()  # seconds per server activity
         active_cells = 0
         with self.context() as ctx:
             t0 = clock()
-            while pending or active:
+            in_flight = False
+            while pending or active or in_flight:
                 now = clock() - t0
                 free, _ = ctx.mem_info()
                 while (pending and requests[pending[0]].arrival <= now
@@ -90,41 +97,58 @@ class Server:
                     ta = clock()
                     slot = ctx.batch_add(requests[i].grid, states[i])
                     spent["admit"] += clock() - ta
-                    active[slot] = [i, 0, clock() - t0]
+                    active[slot] = [i, clock() - t0, 0]
                     active_cells += requests[i].cells
                     free -= need
-                if not active:
-                    sleep(max(0.0, requests[pending[0]].arrival - (clock() - t0)))
+                retiring = []
+                if in_flight and pending and not ctx.batch_poll():
+                    # keep admitting arrivals while the chunk runs
+                    nxt = requests[pending[0]].arrival - (clock() - t0)
+                    sleep(min(nxt, self.poll_s) if nxt > 0.0 else self.poll_s)
                     continue
-                if not tuned and len(active) > 1:
+                if in_flight:
                     ta = clock()
-                    ctx.batch_autotune(probe_steps=2)
-                    spent["autotune"] += clock() - ta
-                    tuned = True
-                chunk = min(self.check_every,
-                            min(self.max_steps - a[1] for a in active.values()))
-                ta = clock()
-                ctx.batch_step(chunk)
-                spent["step"] += clock() - ta
-                for slot in list(active):
-                    i, steps, admitted = active[slot]
-                    steps += chunk
-                    q = ctx.batch_query(slot)
-                    conv = q["residual"] <= self.tol
-                    if conv or steps >= self.max_steps:
+                    ctx.batch_wait()
+                    spent["step"] += clock() - ta
+                    in_flight = False
+                    for slot in list(active):
+                        q = ctx.batch_query(slot)
+                        active[slot][2] = q["steps"]
+                        if q["steps"] == 0:
+                            continue  # admitted while the chunk ran
+                        conv = q["residual"] <= self.tol
+                        if conv or q["steps"] >= self.max_steps:
+                            ctx.batch_remove(slot)  # readable until the next wait
+                            i, admitted, steps = active.pop(slot)
+                            active_cells -= requests[i].cells
+                            retiring.append((slot, i, admitted, steps, q["residual"], conv))
+                if active:
+                    if not tuned and len(active) > 1:
                         ta = clock()
-                        cp, t = ctx.batch_download(slot)
-                        ctx.batch_remove(slot)
-                        spent["retire"] += clock() - ta
-                        del active[slot]
-                        active_cells -= requests[i].cells
-                        done.append({"rid": requests[i].rid, "arrival": requests[i].arrival,
-                                     "admitted": admitted, "done": clock() - t0,
-                                     "steps": steps, "residual": q["residual"],
-                                     "converged": conv, "cells": requests[i].cells,
-                                     "t": t, "result": cp if keep_results else None})
-                    else:
-                        active[slot][1] = steps
+                        ctx.batch_autotune(probe_steps=2)
+                        spent["autotune"] += clock() - ta
+                        tuned = True
+                    chunk = min(self.check_every,
+                                min(self.max_steps - a[2] for a in active.values()))
+                    ctx.batch_step_async(chunk)
+                    in_flight = True
+                for slot, i, admitted, steps, res, conv in retiring:
+                    ta = clock()
+                    cp, t = ctx.batch_download(slot)  # overlaps the chunk just started
+                    spent["retire"] += clock() - ta
+                    done.append({"rid": requests[i].rid, "arrival": requests[i].arrival,
+                                 "admitted": admitted, "done": clock() - t0, "steps": steps,
+                                 "residual": res, "converged": conv, "cells": requests[i].cells,
+                                 "t": t, "result": cp if keep_results else None})
+                if not active and not in_flight and pending:
+                    wait = requests[pending[0]].arrival - (clock() - t0)
+                    if wait > 0.0:
+                        sleep(wait)
+                    elif retiring:
+                        ctx.batch_wait()  # frees the retired requests' memory; admit again
+                    else:  # arrived, nothing running, still not admitted
+                        raise RuntimeError(f"request {requests[pending[0]].rid} does not fit in "
+                                           "device memory")
             wall = clock() - t0
         stats = _latency_stats(done, wall)
         stats["seconds_in"] = dict(spent)

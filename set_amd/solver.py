@@ -38,7 +38,8 @@ EXPORTS = (
     "setmhd_upload_mesh", "setmhd_upload_structured", "setmhd_set_dt", "setmhd_dt_from_state",
     "setmhd_step", "setmhd_step_timed", "setmhd_sync", "setmhd_download",
     "setmhd_download_potok", "setmhd_sizes", "setmhd_residual", "setmhd_last_error",
-    "setmhd_batch_add", "setmhd_batch_remove", "setmhd_batch_step", "setmhd_batch_query",
+    "setmhd_batch_add", "setmhd_batch_remove", "setmhd_batch_step", "setmhd_batch_step_async",
+    "setmhd_batch_poll", "setmhd_batch_wait", "setmhd_batch_query",
     "setmhd_batch_download", "setmhd_batch_autotune", "setmhd_mem_info",
 )
 
@@ -93,6 +94,9 @@ def load():
     L.setmhd_batch_add.argtypes = [P, I32, I32, I32, PD, PD, D, ctypes.POINTER(I32)]
     L.setmhd_batch_remove.argtypes = [P, I32]
     L.setmhd_batch_step.argtypes = [P, I32]
+    L.setmhd_batch_step_async.argtypes = [P, I32]
+    L.setmhd_batch_wait.argtypes = [P]
+    L.setmhd_batch_poll.argtypes = [P, ctypes.POINTER(I32)]
     L.setmhd_batch_query.argtypes = [P, I32, I64P, PD, PD, PD]
     L.setmhd_batch_download.argtypes = [P, I32, PD, PD]
     L.setmhd_batch_autotune.argtypes = [P, I32, ctypes.POINTER(I32), ctypes.POINTER(I32),
@@ -249,11 +253,24 @@ class Context:
         return slot.value
 
     def batch_remove(self, slot):
+        """Retires a slot; its state stays readable until the next wait."""
         self._check(self._L.setmhd_batch_remove(self._h, int(slot)))
-        self._slot_cells.pop(slot, None)
 
     def batch_step(self, nsteps=1):
         self._check(self._L.setmhd_batch_step(self._h, int(nsteps)))
+
+    def batch_step_async(self, nsteps=1):
+        """Queues nsteps steps and returns; see setmhd_batch_step_async."""
+        self._check(self._L.setmhd_batch_step_async(self._h, int(nsteps)))
+
+    def batch_wait(self):
+        self._check(self._L.setmhd_batch_wait(self._h))
+
+    def batch_poll(self):
+        """True when the queued steps have finished (never blocks)."""
+        d = I32()
+        self._check(self._L.setmhd_batch_poll(self._h, ctypes.byref(d)))
+        return bool(d.value)
 
     def batch_query(self, slot):
         n, t, dt, r = ctypes.c_int64(), D(), D(), D()

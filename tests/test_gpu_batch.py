@@ -77,6 +77,34 @@ def test_admission_and_retirement_mid_run():
         assert np.all(e <= 1e-12), e
 
 
+def test_async_step_overlaps_admission_and_download():
+    case("batch step_async with an admission and a retired download in flight")
+    gs = grids()
+    (gA, WA), (gB, WB), (gC, WC) = gs[0], gs[3], gs[2]
+    with solver.Context(solver.cartesian_config()) as c:
+        a = c.batch_add(gA, WA)
+        b = c.batch_add(gB, WB)
+        c.batch_step(10)
+        c.batch_remove(a)            # readable until the next wait
+        c.batch_step_async(7)        # b runs
+        cc = c.batch_add(gC, WC)     # uploaded beside it, joins the next step
+        cpA, _ = c.batch_download(a)
+        qa = c.batch_query(a)
+        qc = c.batch_query(cc)
+        c.batch_wait()
+        with pytest.raises(solver.SetMHDError):
+            c.batch_download(a)      # freed by the wait
+        c.batch_step(5)
+        cpB, _ = c.batch_download(b)
+        cpC, _ = c.batch_download(cc)
+        assert qa["steps"] == 10 and qc["steps"] == 0
+        assert c.batch_query(b)["steps"] == 22 and c.batch_query(cc)["steps"] == 5
+    for cp, (g, W), n in ((cpA, (gA, WA), 10), (cpB, (gB, WB), 22), (cpC, (gC, WC), 5)):
+        ref, _ = alone(g, W, n)
+        e = rel_err(cons(cp), cons(ref), "inf")
+        assert np.all(e <= 1e-12), (n, e)
+
+
 def test_autotune_restores_state():
     case("batch autotune restores state")
     gs = grids()

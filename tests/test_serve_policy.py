@@ -22,12 +22,16 @@ class SimClock:
 
 class ModelContext:
     """setmhd_batch_* model: a request of n cells converges after conv[n]
-    steps; stepping costs cells / rate seconds of the shared clock."""
+    steps; a queued step of the slots admitted at launch ends cells / rate
+    seconds after it started on the shared clock; removed slots free their
+    memory at the next wait, as the library does."""
 
     def __init__(self, clock, free_bytes, conv, rate=1e9):
         self.clock, self.free, self.conv, self.rate = clock, free_bytes, conv, rate
         self.slots = {}
+        self.retired = {}
         self.log = []
+        self.flight = None  # (end time, steps, slots)
 
     def __enter__(self):
         return self
@@ -39,30 +43,44 @@ class ModelContext:
         return self.free, 1 << 40
 
     def batch_add(self, grid, state):
-        s = max(self.slots, default=-1) + 1
+        s = max(list(self.slots) + list(self.retired), default=-1) + 1
         self.slots[s] = [grid.n_cell, 0]
         self.free -= grid.n_cell * serve.BYTES_PER_CELL
         self.log.append(("add", grid.n_cell, self.clock()))
         return s
 
     def batch_remove(self, s):
-        n, _ = self.slots.pop(s)
-        self.free += n * serve.BYTES_PER_CELL
-        self.log.append(("remove", n, self.clock()))
+        assert self.flight is None or s not in self.flight[2]
+        self.retired[s] = self.slots.pop(s)
+        self.log.append(("remove", self.retired[s][0], self.clock()))
 
-    def batch_step(self, n):
+    def batch_step_async(self, n):
+        assert self.flight is None
         cells = sum(c for c, _ in self.slots.values())
-        self.clock.t += n * cells / self.rate
-        for v in self.slots.values():
-            v[1] += n
+        self.flight = (self.clock() + n * cells / self.rate, n, set(self.slots))
+
+    def batch_poll(self):
+        return self.flight is None or self.clock() >= self.flight[0]
+
+    def batch_wait(self):
+        if self.flight:
+            end, n, ss = self.flight
+            self.clock.t = max(self.clock.t, end)
+            for s in ss:
+                self.slots[s][1] += n
+            self.flight = None
+        for n, _ in self.retired.values():
+            self.free += n * serve.BYTES_PER_CELL
+        self.retired.clear()
 
     def batch_query(self, s):
-        n, steps = self.slots[s]
+        assert self.flight is None or s not in self.flight[2]
+        n, steps = self.slots.get(s) or self.retired[s]
         return {"steps": steps, "t": 0.0, "dt": 1e-4,
                 "residual": 1e-7 if steps >= self.conv[n] else 1e-3}
 
     def batch_download(self, s):
-        return np.zeros((self.slots[s][0], 16)), 0.0
+        return np.zeros(((self.slots.get(s) or self.retired[s])[0], 16)), 0.0
 
     def batch_autotune(self, probe_steps=2):
         return (32, 8, 2)
@@ -128,6 +146,16 @@ def test_active_cell_cap_queues_in_arrival_order():
         live += e[1] if e[0] == "add" else -e[1]
         peak = max(peak, live)
     assert peak == 2000 and stats["solves"] == 3
+
+
+def test_arrival_during_a_chunk_is_admitted_before_it_ends():
+    # 10^7-cell request: a 50-step chunk takes 0.5 s of model time; the second
+    # request arrives 0.1 s in and is uploaded while that chunk runs
+    reqs = requests([10 ** 7, 1000], [0.0, 0.1])
+    done, stats, ctx = run(reqs, 1 << 40, {10 ** 7: 200, 1000: 100})
+    adds = [e for e in ctx.log if e[0] == "add"]
+    assert adds[1][2] < 0.5
+    assert stats["solves"] == 2 and stats["converged"] == 2
 
 
 def test_late_arrival_waits_and_latency_stats():
