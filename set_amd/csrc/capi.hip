@@ -98,8 +98,7 @@ struct SlotScal {
   double resid[2];
   double resid_last[2];
   int nstep;
-  int done;
-  int pad[2];
+  int pad[3];
 };
 
 namespace {
