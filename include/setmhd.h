@@ -52,6 +52,8 @@ enum setmhd_status {
 #define SETMHD_F_AXISYM 2      /* cylindrical y-sources /rr(2) (cuf_solvers.cuf:706-736)        */
 #define SETMHD_F_SOURCE_MK 4   /* charge-exchange source (cuf_kernel.cuf:215-288)               */
 #define SETMHD_F_HELIO_HACKS 8 /* LAX wedge :296-298, u<=-0.1 clamp :320-322, v clamp :783-785 */
+#define SETMHD_F_FAST 16       /* drop-in kernels with FMA contraction and reciprocal-based
+                                  division: same algorithm, agrees to rounding, not bit for bit */
 
 #define SETMHD_RIEMANN_LAX 0
 #define SETMHD_RIEMANN_HLL 1

@@ -312,6 +312,16 @@ int grid_kchunk(const setmhd_ctx* c, int nx, int ny, int nz) {
 int tile_x(const setmhd_ctx* c) { return c->cfg.tile_x > 0 ? c->cfg.tile_x : 32; }
 int tile_y(const setmhd_ctx* c) { return c->cfg.tile_y > 0 ? c->cfg.tile_y : 8; }
 
+// drop-in kernels: the exact build, or the fast one under SETMHD_F_FAST
+template <class... A>
+hipError_t face_pass(const setmhd_ctx* c, A... a) {
+  return (c->cfg.flags & SETMHD_F_FAST) ? fast::launch_face_pass(a...) : exact::launch_face_pass(a...);
+}
+template <class... A>
+hipError_t cell_pass(const setmhd_ctx* c, A... a) {
+  return (c->cfg.flags & SETMHD_F_FAST) ? fast::launch_cell_pass(a...) : exact::launch_cell_pass(a...);
+}
+
 // one structured step; dt-only scan is mode -1 (no write)
 int structured_step(setmhd_ctx* c, double* resid) {
   FusedArgs a{};
@@ -356,19 +366,19 @@ int unstructured_step(setmhd_ctx* c) {
   DtRing r = ring(c);
   hipError_t e;
   if (c->cfg.time_scheme == SETMHD_TIME_EULER) {  // cuf_kernel.cuf:837-846
-    e = launch_face_pass(c->stream, c->um, k, c->d_W, c->d_potok, r, cur, 1, 1, c->d_err);
+    e = face_pass(c, c->stream, c->um, k, c->d_W, c->d_potok, r, cur, 1, 1, c->d_err);
     if (e == hipSuccess)
-      e = launch_cell_pass(c->stream, c->um, k, 0, c->d_W, c->d_W2, c->d_H, c->d_potok, r, cur,
+      e = cell_pass(c, c->stream, c->um, k, 0, c->d_W, c->d_W2, c->d_H, c->d_potok, r, cur,
                            c->d_err);
   } else {  // :847-877
-    e = launch_face_pass(c->stream, c->um, k, c->d_W, c->d_potok, r, cur, 1, 1, c->d_err);
+    e = face_pass(c, c->stream, c->um, k, c->d_W, c->d_potok, r, cur, 1, 1, c->d_err);
     if (e == hipSuccess)
-      e = launch_cell_pass(c->stream, c->um, k, 1, c->d_W, c->d_W2, c->d_H, c->d_potok, r, cur,
+      e = cell_pass(c, c->stream, c->um, k, 1, c->d_W, c->d_W2, c->d_H, c->d_potok, r, cur,
                            c->d_err);
     if (e == hipSuccess)
-      e = launch_face_pass(c->stream, c->um, k, c->d_W2, c->d_potok, r, cur, 0, 0, c->d_err);
+      e = face_pass(c, c->stream, c->um, k, c->d_W2, c->d_potok, r, cur, 0, 0, c->d_err);
     if (e == hipSuccess)
-      e = launch_cell_pass(c->stream, c->um, k, 2, c->d_W, c->d_W2, c->d_H, c->d_potok, r, cur,
+      e = cell_pass(c, c->stream, c->um, k, 2, c->d_W, c->d_W2, c->d_H, c->d_potok, r, cur,
                            c->d_err);
   }
   if (e != hipSuccess) return fail(c, SETMHD_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
@@ -636,7 +646,7 @@ int setmhd_dt_from_state(setmhd_ctx* c, double* dt_out) {
   HIPCHK(c, hipMemcpyAsync(c->d_scal, h, sizeof h, hipMemcpyHostToDevice, c->stream));
   hipError_t e;
   if (c->kind == 1) {
-    e = launch_face_pass(c->stream, c->um, ucfg(c->cfg), c->d_W, c->d_potok, ring(c), cur, 1, 0,
+    e = face_pass(c, c->stream, c->um, ucfg(c->cfg), c->d_W, c->d_potok, ring(c), cur, 1, 0,
                          c->d_err);
   } else {
     FusedArgs a{};

@@ -24,11 +24,20 @@ struct UCfg {
   double cfl, dt_cap, r_skip, r_hold;
 };
 
+namespace exact {
 hipError_t launch_face_pass(hipStream_t s, const UMesh& m, const UCfg& c, const double* S,
                             double* potok, DtRing dt, int cur, int reduce, int book, int* err);
 hipError_t launch_cell_pass(hipStream_t s, const UMesh& m, const UCfg& c, int stage, double* W,
                             double* W2, const double* H, const double* potok, DtRing dt, int cur,
                             int* err);
+}  // namespace exact
+namespace fast {
+hipError_t launch_face_pass(hipStream_t s, const UMesh& m, const UCfg& c, const double* S,
+                            double* potok, DtRing dt, int cur, int reduce, int book, int* err);
+hipError_t launch_cell_pass(hipStream_t s, const UMesh& m, const UCfg& c, int stage, double* W,
+                            double* W2, const double* H, const double* potok, DtRing dt, int cur,
+                            int* err);
+}  // namespace fast
 
 // A uniform Cartesian box: state is FP64 conserved, one field of nc
 // doubles per variable (rho, mx, my, mz, E, bx, by, bz), x fastest.
