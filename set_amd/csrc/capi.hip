@@ -322,6 +322,40 @@ hipError_t cell_pass(const setmhd_ctx* c, A... a) {
   return (c->cfg.flags & SETMHD_F_FAST) ? fast::launch_cell_pass(a...) : exact::launch_cell_pass(a...);
 }
 
+// The uniform box of nx x ny x nz cells spanning box[6] (x0,x1,y0,y1,z0,z1;
+// z ignored in 2-D), with the run's CFL number, dt cap and held radius.
+SGrid make_sgrid(const setmhd_config& cfg, int nx, int ny, int nz, const double* box) {
+  SGrid g{};
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  g.nc = (long long)nx * ny * nz;
+  g.x0 = box[0];
+  g.y0 = box[2];
+  g.z0 = nz > 1 ? box[4] : 0.0;
+  g.dx = (box[1] - box[0]) / nx;
+  g.dy = (box[3] - box[2]) / ny;
+  g.dz = nz > 1 ? (box[5] - box[4]) / nz : 1.0;
+  if (nz > 1) {
+    g.ax = g.dy * g.dz;
+    g.ay = g.dx * g.dz;
+    g.az = g.dx * g.dy;
+    g.inv_vol = 1.0 / (g.dx * g.dy * g.dz);
+  } else {
+    g.ax = g.dy;
+    g.ay = g.dx;
+    g.az = 0.0;
+    g.inv_vol = 1.0 / (g.dx * g.dy);
+  }
+  g.distx = 0.5 * g.dx;
+  g.disty = 0.5 * g.dy;
+  g.distz = 0.5 * g.dz;
+  g.cfl = cfg.cfl;
+  g.dt_cap = cfg.dt_cap;
+  g.r_hold = cfg.r_hold_cell;
+  return g;
+}
+
 // one structured step; dt-only scan is mode -1 (no write)
 int structured_step(setmhd_ctx* c, double* resid) {
   FusedArgs a{};
@@ -583,36 +617,7 @@ int setmhd_upload_structured(setmhd_ctx* c, int32_t nx, int32_t ny, int32_t nz, 
   HIPCHK(c, hipMemcpyAsync(c->d_U2, c->d_U, (size_t)nc * 8 * 8, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   hipFree(stage);
-  SGrid& g = c->sg;
-  g = SGrid{};
-  g.nx = nx;
-  g.ny = ny;
-  g.nz = nz;
-  g.nc = nc;
-  g.x0 = box[0];
-  g.y0 = box[2];
-  g.z0 = box[4];
-  g.dx = (box[1] - box[0]) / nx;
-  g.dy = (box[3] - box[2]) / ny;
-  g.dz = nz > 1 ? (box[5] - box[4]) / nz : 1.0;
-  if (nz > 1) {
-    g.ax = g.dy * g.dz;
-    g.ay = g.dx * g.dz;
-    g.az = g.dx * g.dy;
-    g.inv_vol = 1.0 / (g.dx * g.dy * g.dz);
-  } else {
-    g.z0 = 0.0;
-    g.ax = g.dy;
-    g.ay = g.dx;
-    g.az = 0.0;
-    g.inv_vol = 1.0 / (g.dx * g.dy);
-  }
-  g.distx = 0.5 * g.dx;
-  g.disty = 0.5 * g.dy;
-  g.distz = 0.5 * g.dz;
-  g.cfl = c->cfg.cfl;
-  g.dt_cap = c->cfg.dt_cap;
-  g.r_hold = c->cfg.r_hold_cell;
+  c->sg = make_sgrid(c->cfg, nx, ny, nz, box);
   c->kchunk = grid_kchunk(c, nx, ny, nz);
   c->cur_buf = 0;
   c->n_cell = nc;

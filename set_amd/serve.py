@@ -27,7 +27,7 @@ import numpy as np
 from . import solver
 
 # device bytes per cell of an admitted grid: two FP64 state buffers,
-# Cell_par(9:16) and the upload staging of Cell_par (16 doubles)
+# Cell_par(9:16), and room for the library's Cell_par staging (16 doubles)
 BYTES_PER_CELL = 64 + 64 + 64 + 128
 
 
