@@ -163,15 +163,16 @@ __device__ __forceinline__ bool held_cell(const SGrid& g, double xc, double yc, 
   return __dsqrt_rn(r2) < g.r_hold;
 }
 
-// Applies the face sum acc to cell idx.  The cell's own state is read back
-// from the input (an L2 hit: it was read one plane earlier) rather than kept
-// live in registers across the march.
+// Applies the face sum acc to cell idx, whose state U^n is Uold.  The
+// z-march reads U^n of the plane below back from the input (an L2 hit: it
+// was read one plane earlier) at the start of the plane's face phase, so the
+// load is in flight while the faces are formed, rather than keeping it live
+// in registers across the march.
 template <int MODE>
-__device__ __forceinline__ void finalize(const FusedArgs& a, double dt, unsigned idx, bool held,
-                                         const double acc[8], double& rs_d, double& rs_r) {
+__device__ __forceinline__ void finalize_with(const FusedArgs& a, double dt, unsigned idx, bool held,
+                                              const double acc[8], const double Uold[8],
+                                              double& rs_d, double& rs_r) {
   const SGrid& g = a.g;
-  double Uold[8];
-  load8(a.Uf, g.nc, idx, Uold);
   if (held) {
     if (MODE != kModeHeun2) {
 #pragma unroll
@@ -207,6 +208,15 @@ __device__ __forceinline__ void finalize(const FusedArgs& a, double dt, unsigned
     Un[4] = SETMHD_F4(0.0000001) / (kGamma - 1.0) + 0.5 * m2 * frcp(Un[0]) + 0.5 * b2;
 #pragma unroll
   for (int v = 0; v < 8; v++) (a.Uo + v * g.nc)[idx] = Un[v];
+}
+
+// finalize_with, loading U^n of cell idx here
+template <int MODE>
+__device__ __forceinline__ void finalize(const FusedArgs& a, double dt, unsigned idx, bool held,
+                                         const double acc[8], double& rs_d, double& rs_r) {
+  double Uold[8];
+  load8(a.Uf, a.g.nc, idx, Uold);
+  finalize_with<MODE>(a, dt, idx, held, acc, Uold, rs_d, rs_r);
 }
 
 // One tile (TX x TY cells, all planes of one z-chunk in 3-D) of one grid:
@@ -378,6 +388,8 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
       derive<R, false>(Uh, qh);
       put_q(sm, hly, hlx, qh);
     }
+    double Uprev[8];  // U^n of the cell below, updated once the z-face between them is known
+    if (DIM == 3) load8(a.Uf, g.nc, k > k0 ? idx - plane : idx, Uprev);
     __syncthreads();
     // every wave has read the staged plane: stage plane k+1 (the epilogue's
     // plane when k+1 == k1)
@@ -440,7 +452,7 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
 #pragma unroll
         for (int v = 0; v < 8; v++) accp[v] = accp[v] + fz[v] * g.az;
         const double zcen = __dadd_rn(g.z0, __dmul_rn((double)(k - 1) + 0.5, g.dz));
-        finalize<MODE>(a, dt, idx - plane, held_cell(g, xc, yc, zcen), accp, rs_d, rs_r);
+        finalize_with<MODE>(a, dt, idx - plane, held_cell(g, xc, yc, zcen), accp, Uprev, rs_d, rs_r);
       }
 #pragma unroll
       for (int v = 0; v < 8; v++) accp[v] = acc[v];
