@@ -219,10 +219,6 @@ __device__ __forceinline__ void finalize(const FusedArgs& a, double dt, unsigned
   finalize_with<MODE>(a, dt, idx, held, acc, Uold, rs_d, rs_r);
 }
 
-// One tile (TX x TY cells, all planes of one z-chunk in 3-D) of one grid:
-// stage the planes, form the faces, update the cells, and fold the tile's
-// CFL candidate into the grid's next dt slot.  book: this tile also does the
-// step's bookkeeping (t += dt, re-arm the dt slot after next).
 // Raw FP64 state of the own cells of one plane of a tile, [field][row][x],
 // filled by LDS DMA (global_load_lds).
 template <int TX, int TY>
@@ -271,6 +267,10 @@ struct TileRed {
   double dt, rd, rr;
 };
 
+// One tile (TX x TY cells, all planes of one z-chunk in 3-D) of one grid:
+// stage the planes, form the faces, update the cells, and fold the tile's
+// CFL candidate into the grid's next dt slot.  book: this tile also does the
+// step's bookkeeping (t += dt, re-arm the dt slot after next).
 template <class R, int TX, int TY, int DIM, int MODE, bool DMA = false>
 __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool book,
                                               Smem<R, TX, TY>& sm, double* red,
