@@ -231,6 +231,43 @@ def test_mixed_plane_dma_matches_register_path(nx):
     assert np.all(e <= 1e-9), e
 
 
+def test_mixed_heun_close_to_fp64_heun():
+    case("structured 3-D 48x40x36 mixed Heun vs FP64 Heun steps=20")
+    g = mesh.grid_3d(48, 40, 36)
+    W = mesh.heliosphere_state(g, wind=mesh.Wind(r_in=0.3))
+    ref, tr, _ = gpu_run(g, W, 20, time_scheme=1)
+    cp, tm, _ = gpu_run(g, W, 20, precision=1, time_scheme=1)
+    e = rel_err(cons(cp), cons(ref), "l1")
+    print("mixed Heun rel L1 per var", e)
+    assert e[0] <= 1e-5, e
+    assert tm == pytest.approx(tr, rel=1e-5)
+
+
+def test_contexts_in_two_threads_are_independent():
+    """Two contexts stepping at the same time from two host threads (each
+    owns its stream and device state) give the results each gives alone."""
+    import threading
+    case("two contexts in two threads")
+    g1, g2 = mesh.grid_3d(40, 36, 30), mesh.grid_2d(300, 150)
+    W1 = mesh.heliosphere_state(g1, wind=mesh.Wind(r_in=0.3))
+    W2 = mesh.heliosphere_state(g2)
+    ref1, _, _ = gpu_run(g1, W1, 40)
+    ref2, _, _ = gpu_run(g2, W2, 40, precision=1)
+    out = {}
+
+    def work(key, g, W, precision):
+        out[key] = gpu_run(g, W, 40, precision=precision)[0]
+
+    ts = [threading.Thread(target=work, args=(1, g1, W1, 0)),
+          threading.Thread(target=work, args=(2, g2, W2, 1))]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    np.testing.assert_array_equal(out[1], ref1)
+    np.testing.assert_array_equal(out[2], ref2)
+
+
 def test_free_stream_256cube_exact():
     """Full size: a uniform state is a fixed point, bit for bit (every face
     sees the same two states, so every face sum cancels exactly)."""
