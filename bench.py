@@ -113,7 +113,8 @@ def batch_form(precision, steps, seq_steps):
             seq += (time.perf_counter() - t0) / seq_steps
     value = cells * steps / wall
     return {"value": value, "grids": len(grids), "cells": cells, "steps": steps,
-            "ms_per_step": 1e3 * wall / steps, "tile": list(tile),
+            "ms_per_step": 1e3 * wall / steps,
+            "launch_2d": list(tile[2]),  # tile_x, tile_y, blocks per CU (0: one per tile)
             "hbm_frac": value * 128 / (PEAK_HBM_GBS * 1e9),
             "sequential_value": cells / seq, "speedup_vs_one_at_a_time": (cells / seq and value / (cells / seq))}
 

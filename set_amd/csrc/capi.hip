@@ -87,7 +87,10 @@ struct setmhd_ctx {
   hipGraph_t graph1 = nullptr, graphN = nullptr;
   hipGraphExec_t exec1 = nullptr, execN = nullptr;
   int graph_steps = 8;
-  int bt_x = 32, bt_y = 8, bt_bpc = 2;  // batch tile shape, blocks per CU
+  // batch launch per dimensionality ([0] 2-D, [1] 3-D): tile shape and
+  // blocks per CU (0: one block per tile)
+  int bt_x[2] = {32, 32}, bt_y[2] = {8, 8}, bt_bpc[2] = {2, 0};
+  hipEvent_t ev_mid = nullptr;  // autotune: splits a probe between the two launches
   int n_cu = 256;
 };
 
@@ -492,6 +495,7 @@ void setmhd_destroy(setmhd_ctx* c) {
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->ev_side) hipEventDestroy(c->ev_side);
   if (c->ev_step) hipEventDestroy(c->ev_step);
+  if (c->ev_mid) hipEventDestroy(c->ev_mid);
   if (c->side) hipStreamDestroy(c->side);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;

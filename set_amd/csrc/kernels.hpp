@@ -89,7 +89,8 @@ struct BatchLaunch {
   const int* counts;       // device [2]: tiles, grids in use
   const int* blk_first;    // [nblocks + 1] tile range of each block, split by cost
   int cap_tiles, cap_grids;
-  int nblocks;             // persistent grid size
+  int nblocks;             // persistent grid size, or the tile count when per_tile
+  int per_tile;            // 1: one block per tile (k_fused_tiles), 0: persistent blocks
   int dim;                 // 2 or 3: all grids of one launch share it
 };
 

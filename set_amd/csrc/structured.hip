@@ -606,6 +606,40 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(R
   if (threadIdx.x == 0 && cur >= 0) flush();
 }
 
+// Many grids, one block per tile (the other batch launch shape): the
+// hardware balances tiles of unequal cost as blocks finish, as for a single
+// grid, with the same XCD-contiguous tile ranges; the launch size is the
+// tile count, so the captured graph is re-made when that changes.
+template <class R, int TX, int TY, int DIM>
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 2))) void k_fused_tiles(
+    const BatchGrid* __restrict__ grids, const int* __restrict__ tile_grid,
+    const int* __restrict__ tile_first) {
+  __shared__ Smem<R, TX, TY> sm;
+  __shared__ double red[3 * TX * TY / 64];
+  int t = blockIdx.x;
+  const int nblk = gridDim.x;
+  if ((nblk & 7) == 0) t = (t & 7) * (nblk >> 3) + (t >> 3);
+  const int gi = tile_grid[t];
+  const BatchGrid& bg = grids[gi];
+  const int n = *bg.nstep;
+  FusedArgs a;
+  a.g = bg.g;
+  a.Uf = bg.buf[n & 1];
+  a.Un = a.Uf;
+  a.Uo = bg.buf[(n & 1) ^ 1];
+  a.dt = bg.dt;
+  a.cur = n % 3;
+  a.kchunk = bg.kchunk;
+  a.resid = bg.resid;
+  a.err = bg.err;
+  const TileRed r = fused_tile<R, TX, TY, DIM, kModeEuler>(a, t - tile_first[gi], false, sm, red);
+  if (threadIdx.x == 0) {
+    atomic_min_pos(&a.dt.slot[(a.cur + 1) % 3], r.dt);
+    atomicAdd(&a.resid[0], r.rd);
+    atomicAdd(&a.resid[1], r.rr);
+  }
+}
+
 // Closes one batch step for each grid: t += dt (cuf_kernel.cuf:792-794),
 // re-arms the dt slot after next (:790), publishes the step's density
 // residual and advances the grid's step index.
@@ -662,12 +696,20 @@ static hipError_t launch_prec(hipStream_t s, const FusedArgs& a, int mode, int t
 
 template <class R, int TX, int TY>
 static hipError_t launch_batch_tile(hipStream_t s, const BatchLaunch& L) {
-  if (L.dim == 3)
+  if (L.per_tile) {
+    if (L.dim == 3)
+      hipLaunchKernelGGL((k_fused_tiles<R, TX, TY, 3>), dim3(L.nblocks), dim3(TX * TY), 0, s, L.grids,
+                         L.tile_grid, L.tile_first);
+    else
+      hipLaunchKernelGGL((k_fused_tiles<R, TX, TY, 2>), dim3(L.nblocks), dim3(TX * TY), 0, s, L.grids,
+                         L.tile_grid, L.tile_first);
+  } else if (L.dim == 3) {
     hipLaunchKernelGGL((k_fused_batch<R, TX, TY, 3>), dim3(L.nblocks), dim3(TX * TY), 0, s, L.grids,
                        L.tile_grid, L.tile_first, L.blk_first);
-  else
+  } else {
     hipLaunchKernelGGL((k_fused_batch<R, TX, TY, 2>), dim3(L.nblocks), dim3(TX * TY), 0, s, L.grids,
                        L.tile_grid, L.tile_first, L.blk_first);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_batch_book, dim3((L.cap_grids + 63) / 64), dim3(64), 0, s, L.grids, L.counts);

@@ -40,7 +40,7 @@ EXPORTS = (
     "setmhd_download_potok", "setmhd_sizes", "setmhd_residual", "setmhd_last_error",
     "setmhd_batch_add", "setmhd_batch_remove", "setmhd_batch_step", "setmhd_batch_step_async",
     "setmhd_batch_poll", "setmhd_batch_wait", "setmhd_batch_query",
-    "setmhd_batch_download", "setmhd_batch_autotune", "setmhd_mem_info",
+    "setmhd_batch_download", "setmhd_batch_autotune", "setmhd_batch_set_launch", "setmhd_mem_info",
 )
 
 
@@ -94,6 +94,7 @@ def load():
     L.setmhd_batch_add.argtypes = [P, I32, I32, I32, PD, PD, D, ctypes.POINTER(I32)]
     L.setmhd_batch_remove.argtypes = [P, I32]
     L.setmhd_batch_step.argtypes = [P, I32]
+    L.setmhd_batch_set_launch.argtypes = [P, I32, I32, I32, I32]
     L.setmhd_batch_step_async.argtypes = [P, I32]
     L.setmhd_batch_wait.argtypes = [P]
     L.setmhd_batch_poll.argtypes = [P, ctypes.POINTER(I32)]
@@ -284,11 +285,19 @@ class Context:
         self._check(self._L.setmhd_batch_download(self._h, int(slot), _pd(cp), ctypes.byref(t)))
         return cp, t.value
 
+    def batch_set_launch(self, tile=(32, 8), blocks_per_cu=2, dim=0):
+        """Fixes the batch launch of 2-D (dim 2), 3-D (dim 3) or all (0)
+        grids; blocks_per_cu 0: one block per tile."""
+        self._check(self._L.setmhd_batch_set_launch(self._h, int(dim), int(tile[0]), int(tile[1]),
+                                                    int(blocks_per_cu)))
+
     def batch_autotune(self, probe_steps=10):
-        tx, ty, bpc = I32(), I32(), I32()
-        self._check(self._L.setmhd_batch_autotune(self._h, int(probe_steps), ctypes.byref(tx),
-                                                  ctypes.byref(ty), ctypes.byref(bpc)))
-        return tx.value, ty.value, bpc.value
+        """Picks the fastest launch of the 2-D and of the 3-D grids; returns
+        {2: (tile_x, tile_y, blocks_per_cu), 3: (...)} (blocks_per_cu 0: one
+        block per tile)."""
+        tx, ty, bpc = (I32 * 2)(), (I32 * 2)(), (I32 * 2)()
+        self._check(self._L.setmhd_batch_autotune(self._h, int(probe_steps), tx, ty, bpc))
+        return {2: (tx[0], ty[0], bpc[0]), 3: (tx[1], ty[1], bpc[1])}
 
     def mem_info(self):
         f, t = ctypes.c_int64(), ctypes.c_int64()

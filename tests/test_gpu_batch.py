@@ -38,11 +38,15 @@ def grids():
     return out
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_batch_equals_alone(precision):
-    case(f"batch of 5 (2-D + 3-D) vs alone, precision={precision}")
+@pytest.mark.parametrize("precision,launch", [(0, ((32, 8), 2, 0)), (1, ((32, 8), 2, 0)),
+                                              (0, ((32, 8), 0, 0)), (1, ((64, 4), 0, 0)),
+                                              (0, ((16, 16), 64, 0)), (0, None)])
+def test_batch_equals_alone(precision, launch):
+    case(f"batch of 5 (2-D + 3-D) vs alone, precision={precision}, launch={launch}")
     gs = grids()
     with solver.Context(solver.cartesian_config(precision=precision)) as c:
+        if launch:
+            c.batch_set_launch(*launch)  # else the defaults (2-D persistent, 3-D per tile)
         slots = [c.batch_add(g, W) for g, W in gs]
         c.batch_step(37)
         res = [c.batch_download(s) for s in slots]
@@ -111,8 +115,9 @@ def test_autotune_restores_state():
     with solver.Context(solver.cartesian_config()) as c:
         slots = [c.batch_add(g, W) for g, W in gs]
         c.batch_step(4)
-        tx, ty, bpc = c.batch_autotune(probe_steps=3)
-        assert (tx, ty) in {(32, 8), (16, 16), (64, 4), (16, 8)} and bpc in (1, 2, 4, 8, 64)
+        tuned = c.batch_autotune(probe_steps=3)
+        for tx, ty, bpc in tuned.values():
+            assert (tx, ty) in {(32, 8), (16, 16), (64, 4), (16, 8)} and bpc in (0, 1, 2, 4, 8, 64)
         assert all(c.batch_query(s)["steps"] == 4 for s in slots)
         c.batch_step(6)
         res = [c.batch_download(s)[0] for s in slots]
