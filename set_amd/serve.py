@@ -46,7 +46,7 @@ def _latency_stats(done, wall):
 
 class Server:
     def __init__(self, precision=solver.PREC_FP64, check_every=50, max_steps=5000, tol=1e-6,
-                 reserve_bytes=4 << 30, max_active=512, max_active_cells=None, autotune=True,
+                 reserve_bytes=4 << 30, max_active=512, max_active_cells=None, autotune=False,
                  context=None, poll_s=0.001):
         self.precision = precision
         self.check_every = check_every
@@ -58,6 +58,9 @@ class Server:
         # GPU, more concurrency only stretches every request's latency);
         # a request larger than the cap still runs, alone
         self.max_active_cells = max_active_cells
+        # Off by default: the library's launches (2-D: persistent blocks;
+        # 3-D: one block per tile) measured better on a changing request mix
+        # than a tuning done on the first two requests.
         self.autotune = autotune
         self.poll_s = poll_s  # host sleep while a chunk runs and requests are due
         # context factory (a solver.Context by default; tests pass a model)
