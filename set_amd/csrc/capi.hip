@@ -304,9 +304,15 @@ UCfg ucfg(const setmhd_config& k) {
 // blocks to fill the chip (4 per CU with 32x8 columns), at most 32 planes
 // and at least 8 (each chunk re-reads two boundary planes).  A grid gets the
 // same chunking alone and in a batch, so both compute it identically.
+// planes (3-D) or rows (2-D) a block marches: enough blocks for about four
+// per CU, within [8, 32] planes or [8, 64] rows
 int grid_kchunk(const setmhd_ctx* c, int nx, int ny, int nz) {
   if (c->cfg.kchunk > 0) return c->cfg.kchunk;
-  if (nz <= 1) return 32;
+  if (nz <= 1) {
+    const long long nseg = (nx + ROWN - 1) / ROWN;
+    const long long rc = (ny * nseg + 4LL * c->n_cu - 1) / (4LL * c->n_cu);
+    return (int)std::min(64LL, std::max(8LL, rc));
+  }
   const long long txy = (long long)((nx + 31) / 32) * ((ny + 7) / 8);
   const long long kc = (nz * txy + 4LL * c->n_cu - 1) / (4LL * c->n_cu);
   return (int)std::min(32LL, std::max(8LL, kc));

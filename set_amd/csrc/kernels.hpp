@@ -94,9 +94,14 @@ struct BatchLaunch {
   int dim;                 // 2 or 3: all grids of one launch share it
 };
 
-// tiles of grid g for a TX x TY tile (and kchunk planes per tile in 3-D)
+// 2-D grids march rows: blocks of RW lanes own RW - 2 columns each
+constexpr int RW = 256, ROWN = RW - 2;
+
+// tiles of grid g: 3-D, TX x TY columns x chunks of kchunk planes; 2-D,
+// column segments x chunks of kchunk rows
 inline int tiles_of(const SGrid& g, int tx, int ty, int kchunk) {
-  const int nkz = g.nz > 1 ? (g.nz + kchunk - 1) / kchunk : 1;
+  if (g.nz == 1) return ((g.nx + ROWN - 1) / ROWN) * ((g.ny + kchunk - 1) / kchunk);
+  const int nkz = (g.nz + kchunk - 1) / kchunk;
   return ((g.nx + tx - 1) / tx) * ((g.ny + ty - 1) / ty) * nkz;
 }
 

@@ -26,6 +26,8 @@
 // |centre| < 0.05 (cuf_solvers.cuf:611).  Precision: R = double, or R = float
 // for the face states and Riemann solver with the state, the pressure
 // recovery, the flux accumulation and the dt reduction kept in FP64.
+#include <type_traits>
+
 #include "hlld_fast.hpp"
 #include "kernels.hpp"
 
@@ -527,6 +529,211 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
   return out;
 }
 
+// ---- 2-D: row march --------------------------------------------------------
+// A 2-D block is a segment of RW = 256 columns (lane l holds column
+// x0 + l - 1: lanes 1..RW-2 own their column, lanes 0 and RW-1 are the
+// halo) marching up a chunk of rows.  Per row every lane derives its cell
+// once; lane l forms the x-face between itself and lane l+1 (its own data
+// from registers, the neighbour's from LDS) and the y-face below its cell
+// from the previous row held in registers.  No halo pass and no extra edge
+// pass; the next row is prefetched into registers.  Overlapping segments
+// (254 owned columns each) re-derive one column on each side.
+enum { XR = 0, XU = 1, XB = 4, XPT = 7, XE = 8, XCF = 9, NQX = 10 };
+
+template <class R>
+struct RowSmem {
+  R q[NQX][RW];   // x-face data of the row
+  R fx[8][RW];    // flux of the x-face on the right of lane l (x, y, z components)
+};
+
+template <class R>
+__device__ __forceinline__ Side<R> row_side(const RowSmem<R>& sm, int l) {
+  Side<R> s;
+  s.rho = sm.q[XR][l];
+  s.vn = sm.q[XU][l];
+  s.vt1 = sm.q[XU + 1][l];
+  s.vt2 = sm.q[XU + 2][l];
+  s.bn = sm.q[XB][l];
+  s.bt1 = sm.q[XB + 1][l];
+  s.bt2 = sm.q[XB + 2][l];
+  s.pt = sm.q[XPT][l];
+  s.e = sm.q[XE][l];
+  s.cf = sm.q[XCF][l];
+  return s;
+}
+
+// y-face flux in the face frame (n = y, t1 = z, t2 = x) -> x, y, z components
+template <class R>
+__device__ __forceinline__ void from_y_frame(const R F[8], double f[8]) {
+  f[0] = F[0];
+  f[2] = F[1];
+  f[3] = F[2];
+  f[1] = F[3];
+  f[4] = F[4];
+  f[6] = F[5];
+  f[7] = F[6];
+  f[5] = F[7];
+}
+
+template <class R, int MODE>
+__device__ __forceinline__ TileRed row_tile(const FusedArgs& a, int b, bool book, RowSmem<R>& sm,
+                                            double* red) {
+  constexpr int NW = RW / 64;
+  const SGrid& g = a.g;
+  const int l = threadIdx.x, lane = l & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(l >> 6);
+  const int ntx = (g.nx + ROWN - 1) / ROWN;
+  const int seg = b % ntx, rc = b / ntx;
+  const int c = seg * ROWN + l - 1;
+  const int cc = min(max(c, 0), g.nx - 1);
+  const bool own = l >= 1 && l <= ROWN && c < g.nx;
+  const int j0 = rc * a.kchunk, j1 = min(j0 + a.kchunk, g.ny);
+  const unsigned row = (unsigned)g.nx;
+  const double dt = a.dt.slot[a.cur];
+  if (MODE != kModeHeun2 && book && l == 0) {
+    a.dt.t[0] = a.dt.t[0] + dt;                 // cuf_kernel.cuf:792-794
+    a.dt.slot[(a.cur + 2) % 3] = kDtReset;      // :790, for step n+2
+  }
+  const double xc = __dadd_rn(g.x0, __dmul_rn((double)cc + 0.5, g.dx));
+  double smax[2] = {0.0, 0.0};
+  double rs_d = 0.0, rs_r = 0.0;
+  double accp[8];
+  bool ok = true;
+  Side<R> yp;
+  {  // y-face data of the row below the chunk
+    double U[8];
+    load8(a.Uf, g.nc, (unsigned)cc + row * (unsigned)max(j0 - 1, 0), U);
+    CellQ<R> q;
+    derive<R, false>(U, q);
+    yp = side_of(q, 1);
+  }
+  double Unx[8];
+  load8(a.Uf, g.nc, (unsigned)cc + row * (unsigned)j0, Unx);
+
+  for (int j = j0; j < j1; ++j) {
+    const unsigned idx = (unsigned)cc + row * (unsigned)j;
+    double Uc[8];
+#pragma unroll
+    for (int v = 0; v < 8; v++) Uc[v] = Unx[v];
+    load8(a.Uf, g.nc, (unsigned)cc + row * (unsigned)min(j + 1, g.ny - 1), Unx);  // next row
+    CellQ<R> qc;
+    derive<R, false>(Uc, qc);
+    sm.q[XR][l] = qc.rho;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      sm.q[XU + k][l] = qc.u[k];
+      sm.q[XB + k][l] = qc.b[k];
+    }
+    sm.q[XPT][l] = qc.pt;
+    sm.q[XE][l] = qc.e;
+    sm.q[XCF][l] = qc.cf[0];
+    double Uprev[8];  // U^n of the cell below, updated once the y-face between them is known
+    load8(a.Uf, g.nc, j > j0 ? idx - row : idx, Uprev);
+    __syncthreads();
+
+    if (l < RW - 1) {  // x-face on the right of this lane's cell
+      R F[8], s;
+      ok &= hlld(side_of(qc, 0), row_side(sm, l + 1), F, s);
+      smax[0] = fmax(smax[0], (double)s);
+#pragma unroll
+      for (int v = 0; v < 8; v++) sm.fx[v][l] = F[v];
+    }
+    R Fy[8];
+    const Side<R> yc = side_of(qc, 1);
+    {  // y-face below this lane's cell
+      R s;
+      ok &= hlld(yp, yc, Fy, s);
+      smax[1] = fmax(smax[1], (double)s);
+    }
+    __syncthreads();
+
+    // signed face sum in the face-slot order (x-, x+, y-, y+)
+    double fy[8];
+    from_y_frame(Fy, fy);
+    double acc[8];
+    const int lm = max(l - 1, 0);
+#pragma unroll
+    for (int v = 0; v < 8; v++) {
+      double s = -((double)sm.fx[v][lm] * g.ax);
+      s = s + (double)sm.fx[v][l] * g.ax;
+      acc[v] = s - fy[v] * g.ay;
+    }
+    if (j > j0 && own) {
+#pragma unroll
+      for (int v = 0; v < 8; v++) accp[v] = accp[v] + fy[v] * g.ay;
+      const double ycen = __dadd_rn(g.y0, __dmul_rn((double)(j - 1) + 0.5, g.dy));
+      finalize_with<MODE>(a, dt, idx - row, held_cell(g, xc, ycen, 0.0), accp, Uprev, rs_d, rs_r);
+    }
+#pragma unroll
+    for (int v = 0; v < 8; v++) accp[v] = acc[v];
+    yp = yc;
+  }
+  {  // y-face above the chunk's last row (row min(j1, ny-1), prefetched)
+    CellQ<R> q;
+    derive<R, false>(Unx, q);
+    R Fy[8], s;
+    ok &= hlld(yp, side_of(q, 1), Fy, s);
+    smax[1] = fmax(smax[1], (double)s);
+    if (own) {
+      double fy[8];
+      from_y_frame(Fy, fy);
+#pragma unroll
+      for (int v = 0; v < 8; v++) accp[v] = accp[v] + fy[v] * g.ay;
+      const double ycen = __dadd_rn(g.y0, __dmul_rn((double)(j1 - 1) + 0.5, g.dy));
+      finalize<MODE>(a, dt, (unsigned)cc + row * (unsigned)(j1 - 1), held_cell(g, xc, ycen, 0.0), accp,
+                     rs_d, rs_r);
+    }
+  }
+  if (!ok) atomicOr(a.err, kErrRegion);
+
+  const bool want_rs = a.resid != nullptr && MODE != kModeHeun1;
+  double cand = kDtReset;
+  if (MODE != kModeHeun2) {
+    cand = g.cfl * g.distx / smax[0];
+    cand = fmin(cand, g.cfl * g.disty / smax[1]);
+    cand = fmin(cand, g.dt_cap);
+    cand = wave_min(cand);
+  }
+  if (want_rs) {
+    rs_d = wave_sum(rs_d);
+    rs_r = wave_sum(rs_r);
+  }
+  if (lane == 0) {
+    red[wid] = cand;
+    red[NW + wid] = rs_d;
+    red[2 * NW + wid] = rs_r;
+  }
+  __syncthreads();
+  TileRed out{kDtReset, 0.0, 0.0};
+  if (l == 0) {
+    out = TileRed{red[0], red[NW], red[2 * NW]};
+    for (int w = 1; w < NW; w++) {
+      out.dt = fmin(out.dt, red[w]);
+      out.rd += red[NW + w];
+      out.rr += red[2 * NW + w];
+    }
+  }
+  return out;
+}
+
+// One 2-D grid: one block per segment x row chunk (XCD-contiguous ranges).
+template <class R, int MODE>
+__global__ __launch_bounds__(RW) void k_rows(FusedArgs a) {
+  __shared__ RowSmem<R> sm;
+  __shared__ double red[3 * RW / 64];
+  int b = blockIdx.x;
+  const int nblk = gridDim.x;
+  if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
+  const TileRed r = row_tile<R, MODE>(a, b, blockIdx.x == 0, sm, red);
+  if (threadIdx.x == 0) {
+    if (MODE != kModeHeun2) atomic_min_pos(&a.dt.slot[(a.cur + 1) % 3], r.dt);
+    if (a.resid != nullptr && MODE != kModeHeun1) {
+      atomicAdd(&a.resid[0], r.rd);
+      atomicAdd(&a.resid[1], r.rr);
+    }
+  }
+}
+
 // One grid: one block per tile, tiles remapped so each XCD (blockIdx % 8)
 // gets a contiguous range (neighbouring tiles share halo lines in its L2).
 template <class R, int TX, int TY, int DIM, int MODE, bool DMA>
@@ -552,6 +759,22 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(R
   }
 }
 
+// Shared memory and tile routine of a batch block: the row march for 2-D
+// grids (launched with RW = 32 x 8 threads), the z-march for 3-D grids.
+template <class R, int TX, int TY, int DIM>
+using BlockSmem = std::conditional_t<DIM == 2, RowSmem<R>, Smem<R, TX, TY>>;
+
+template <class R, int TX, int TY, int DIM>
+__device__ __forceinline__ TileRed batch_tile(const FusedArgs& a, int b, BlockSmem<R, TX, TY, DIM>& sm,
+                                              double* red) {
+  if constexpr (DIM == 2) {
+    static_assert(TX * TY == RW, "2-D batch blocks are row segments");
+    return row_tile<R, kModeEuler>(a, b, false, sm, red);
+  } else {
+    return fused_tile<R, TX, TY, DIM, kModeEuler>(a, b, false, sm, red);
+  }
+}
+
 // Many grids, one persistent launch per step.  Block b walks a contiguous
 // range of the global tile list, cut so that every range carries about the
 // same number of planes (ranges in block order: dealing them out per XCD
@@ -564,7 +787,7 @@ template <class R, int TX, int TY, int DIM>
 __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 2))) void k_fused_batch(
     const BatchGrid* __restrict__ grids, const int* __restrict__ tile_grid,
     const int* __restrict__ tile_first, const int* __restrict__ blk_first) {
-  __shared__ Smem<R, TX, TY> sm;
+  __shared__ BlockSmem<R, TX, TY, DIM> sm;
   __shared__ double red[3 * TX * TY / 64];
   // tables on the device: the batch can change between graph replays
   const int t0 = blk_first[blockIdx.x], t1 = blk_first[blockIdx.x + 1];
@@ -595,7 +818,7 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(R
     a.kchunk = bg.kchunk;
     a.resid = bg.resid;
     a.err = bg.err;
-    const TileRed r = fused_tile<R, TX, TY, DIM, kModeEuler>(a, t - tile_first[gi], false, sm, red);
+    const TileRed r = batch_tile<R, TX, TY, DIM>(a, t - tile_first[gi], sm, red);
     if (threadIdx.x == 0) {
       acc.dt = fmin(acc.dt, r.dt);
       acc.rd += r.rd;
@@ -614,7 +837,7 @@ template <class R, int TX, int TY, int DIM>
 __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 2))) void k_fused_tiles(
     const BatchGrid* __restrict__ grids, const int* __restrict__ tile_grid,
     const int* __restrict__ tile_first) {
-  __shared__ Smem<R, TX, TY> sm;
+  __shared__ BlockSmem<R, TX, TY, DIM> sm;
   __shared__ double red[3 * TX * TY / 64];
   int t = blockIdx.x;
   const int nblk = gridDim.x;
@@ -632,7 +855,7 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(R
   a.kchunk = bg.kchunk;
   a.resid = bg.resid;
   a.err = bg.err;
-  const TileRed r = fused_tile<R, TX, TY, DIM, kModeEuler>(a, t - tile_first[gi], false, sm, red);
+  const TileRed r = batch_tile<R, TX, TY, DIM>(a, t - tile_first[gi], sm, red);
   if (threadIdx.x == 0) {
     atomic_min_pos(&a.dt.slot[(a.cur + 1) % 3], r.dt);
     atomicAdd(&a.resid[0], r.rd);
@@ -659,16 +882,21 @@ __global__ void k_batch_book(const BatchGrid* __restrict__ grids, const int* __r
 
 template <class R, int TX, int TY, int DIM, int MODE>
 static hipError_t launch_one(hipStream_t s, const FusedArgs& a) {
-  const int ntx = (a.g.nx + TX - 1) / TX, nty = (a.g.ny + TY - 1) / TY;
-  const int nkz = DIM == 3 ? (a.g.nz + a.kchunk - 1) / a.kchunk : 1;
-  const long long nb = (long long)ntx * nty * nkz;
-  // plane DMA: mixed form, 3-D, 32x8 tiles (the staging fits 3 blocks/CU
-  // beside the face data), rows of even length (16-byte pairs)
-  constexpr bool kDma = sizeof(R) == 4 && DIM == 3 && TX == 32 && TY == 8;
-  if (kDma && a.g.nx % 2 == 0)
-    hipLaunchKernelGGL((k_fused<R, TX, TY, DIM, MODE, kDma>), dim3((unsigned)nb), dim3(TX * TY), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_fused<R, TX, TY, DIM, MODE, false>), dim3((unsigned)nb), dim3(TX * TY), 0, s, a);
+  if constexpr (DIM == 2) {  // 2-D grids march rows whatever the tile shape
+    hipLaunchKernelGGL((k_rows<R, MODE>), dim3((unsigned)tiles_of(a.g, 0, 0, a.kchunk)), dim3(RW), 0, s,
+                       a);
+  } else {
+    const int ntx = (a.g.nx + TX - 1) / TX, nty = (a.g.ny + TY - 1) / TY;
+    const int nkz = (a.g.nz + a.kchunk - 1) / a.kchunk;
+    const long long nb = (long long)ntx * nty * nkz;
+    // plane DMA: mixed form, 32x8 tiles (the staging fits 3 blocks/CU
+    // beside the face data), rows of even length (16-byte pairs)
+    constexpr bool kDma = sizeof(R) == 4 && TX == 32 && TY == 8;
+    if (kDma && a.g.nx % 2 == 0)
+      hipLaunchKernelGGL((k_fused<R, TX, TY, DIM, MODE, kDma>), dim3((unsigned)nb), dim3(TX * TY), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_fused<R, TX, TY, DIM, MODE, false>), dim3((unsigned)nb), dim3(TX * TY), 0, s, a);
+  }
   return hipGetLastError();
 }
 
@@ -696,17 +924,18 @@ static hipError_t launch_prec(hipStream_t s, const FusedArgs& a, int mode, int t
 
 template <class R, int TX, int TY>
 static hipError_t launch_batch_tile(hipStream_t s, const BatchLaunch& L) {
+  constexpr bool rows = TX == 32 && TY == 8;  // 2-D blocks: RW-wide row segments
   if (L.per_tile) {
     if (L.dim == 3)
       hipLaunchKernelGGL((k_fused_tiles<R, TX, TY, 3>), dim3(L.nblocks), dim3(TX * TY), 0, s, L.grids,
                          L.tile_grid, L.tile_first);
-    else
+    else if constexpr (rows)
       hipLaunchKernelGGL((k_fused_tiles<R, TX, TY, 2>), dim3(L.nblocks), dim3(TX * TY), 0, s, L.grids,
                          L.tile_grid, L.tile_first);
   } else if (L.dim == 3) {
     hipLaunchKernelGGL((k_fused_batch<R, TX, TY, 3>), dim3(L.nblocks), dim3(TX * TY), 0, s, L.grids,
                        L.tile_grid, L.tile_first, L.blk_first);
-  } else {
+  } else if constexpr (rows) {
     hipLaunchKernelGGL((k_fused_batch<R, TX, TY, 2>), dim3(L.nblocks), dim3(TX * TY), 0, s, L.grids,
                        L.tile_grid, L.tile_first, L.blk_first);
   }
@@ -718,6 +947,10 @@ static hipError_t launch_batch_tile(hipStream_t s, const BatchLaunch& L) {
 
 hipError_t launch_batch(hipStream_t s, const BatchLaunch& L, int precision, int tile_x, int tile_y) {
   if (L.nblocks <= 0 || L.cap_grids <= 0) return hipSuccess;
+  if (L.dim == 2) {  // row march: the tile shape does not apply
+    tile_x = 32;
+    tile_y = 8;
+  }
 #define SETMHD_BT(T)                                                         \
   if (tile_x == 32 && tile_y == 8) return launch_batch_tile<T, 32, 8>(s, L);   \
   if (tile_x == 16 && tile_y == 16) return launch_batch_tile<T, 16, 16>(s, L); \
@@ -734,7 +967,7 @@ hipError_t launch_batch(hipStream_t s, const BatchLaunch& L, int precision, int 
 
 hipError_t launch_fused(hipStream_t s, const FusedArgs& a, int mode, int precision, int tile_x,
                         int tile_y) {
-  if (a.g.nz > 1 && a.kchunk < 1) return hipErrorInvalidValue;
+  if (a.kchunk < 1) return hipErrorInvalidValue;
   return precision == 0 ? launch_prec<double>(s, a, mode, tile_x, tile_y)
                         : launch_prec<float>(s, a, mode, tile_x, tile_y);
 }
