@@ -210,21 +210,22 @@ int setmhd_batch_query(setmhd_ctx *ctx, int32_t slot, int64_t *steps,
 int setmhd_batch_download(setmhd_ctx *ctx, int32_t slot, double *cell_par,
                           double *t);
 
-/* Times probe_steps steps of the admitted batch for each tile shape
- * (32x8, 16x16, 64x4, 16x8) and launch shape (persistent grids of 1, 2, 4,
- * 8, 64 blocks per CU, or one block per tile, reported as 0), the 2-D and
- * the 3-D launch separately; restores the state after each probe, keeps the
- * fastest of each for this context and caches them per (precision,
- * dimensionalities present, size bucket).  Outputs, if not NULL, are arrays
- * of 2: [0] the 2-D launch, [1] the 3-D launch. */
+/* Times probe_steps steps of the admitted batch for each launch shape
+ * (persistent grids of 1, 2, 4, 8, 64 blocks per CU, or one block per tile,
+ * reported as 0) and, for the 3-D grids, each tile shape (32x8, 16x16,
+ * 64x4, 16x8; 2-D grids march 256-cell row segments whatever the tile and
+ * report 32x8), the 2-D and the 3-D launch separately; restores the state
+ * after each probe, keeps the fastest of each for this context and caches
+ * them per (precision, dimensionalities present, size bucket).  Outputs, if
+ * not NULL, are arrays of 2: [0] the 2-D launch, [1] the 3-D launch. */
 int setmhd_batch_autotune(setmhd_ctx *ctx, int32_t probe_steps,
                           int32_t *tile_x, int32_t *tile_y,
                           int32_t *blocks_per_cu);
 
 /* Sets the batch launch of 2-D (dim 2), 3-D (dim 3) or all (dim 0) grids
- * without timing: tile shape (32x8, 16x16, 64x4 or 16x8) and blocks per CU
- * (1..64 persistent blocks, or 0 for one block per tile).  Defaults: 2-D
- * 32x8 with 2 persistent blocks per CU, 3-D 32x8 with one block per tile. */
+ * without timing: tile shape (32x8, 16x16, 64x4 or 16x8; 3-D grids only,
+ * 2-D grids keep 32x8) and blocks per CU (1..64 persistent blocks, or 0 for
+ * one block per tile).  Defaults: 32x8 with one block per tile. */
 int setmhd_batch_set_launch(setmhd_ctx *ctx, int32_t dim, int32_t tile_x, int32_t tile_y,
                             int32_t blocks_per_cu);
 

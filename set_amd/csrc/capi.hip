@@ -89,7 +89,7 @@ struct setmhd_ctx {
   int graph_steps = 8;
   // batch launch per dimensionality ([0] 2-D, [1] 3-D): tile shape and
   // blocks per CU (0: one block per tile)
-  int bt_x[2] = {32, 32}, bt_y[2] = {8, 8}, bt_bpc[2] = {2, 0};
+  int bt_x[2] = {32, 32}, bt_y[2] = {8, 8}, bt_bpc[2] = {0, 0};
   hipEvent_t ev_mid = nullptr;  // autotune: splits a probe between the two launches
   int n_cu = 256;
 };
@@ -300,18 +300,32 @@ UCfg ucfg(const setmhd_config& k) {
   return UCfg{k.riemann, k.recon, k.flags, k.cfl, k.dt_cap, k.r_skip_face, k.r_hold_cell};
 }
 
-// z-planes per block of a 3-D grid: the configured value, else enough
-// blocks to fill the chip (4 per CU with 32x8 columns), at most 32 planes
-// and at least 8 (each chunk re-reads two boundary planes).  A grid gets the
-// same chunking alone and in a batch, so both compute it identically.
-// planes (3-D) or rows (2-D) a block marches: enough blocks for about four
-// per CU, within [8, 32] planes or [8, 64] rows
-int grid_kchunk(const setmhd_ctx* c, int nx, int ny, int nz) {
+// planes (3-D) or rows (2-D) one block marches, the configured value if set.
+// 3-D: enough blocks for about four per CU with 32x8 columns, within [8, 32]
+// planes (each chunk re-reads two boundary planes).  2-D in a batch: 16
+// rows (the grids fill the chip together).  A 2-D grid alone: a block
+// re-reads one row below its chunk and starts up in about two row-times,
+// and the launch runs in rounds of two resident blocks per CU (the FP64
+// occupancy; a third mixed-form block adds little), so take the chunk in
+// [1, 32] with the fewest row-times over its rounds, (k + 3) x rounds,
+// among those whose blocks fill 3/4 of a round (one row when none does).
+int grid_kchunk(const setmhd_ctx* c, int nx, int ny, int nz, bool batch) {
   if (c->cfg.kchunk > 0) return c->cfg.kchunk;
   if (nz <= 1) {
-    const long long nseg = (nx + ROWN - 1) / ROWN;
-    const long long rc = (ny * nseg + 4LL * c->n_cu - 1) / (4LL * c->n_cu);
-    return (int)std::min(64LL, std::max(8LL, rc));
+    if (batch) return 16;
+    const long long nseg = (nx + ROWN - 1) / ROWN, slots = 2LL * c->n_cu;
+    int best = 1;
+    long long best_cost = -1;
+    for (int k = 1; k <= 32; k++) {
+      const long long nb = nseg * ((ny + k - 1) / k);
+      if (4 * nb < 3 * slots) break;  // nb does not grow with k
+      const long long cost = (nb + slots - 1) / slots * (k + 3);
+      if (best_cost < 0 || cost <= best_cost) {
+        best = k;
+        best_cost = cost;
+      }
+    }
+    return best;
   }
   const long long txy = (long long)((nx + 31) / 32) * ((ny + 7) / 8);
   const long long kc = (nz * txy + 4LL * c->n_cu - 1) / (4LL * c->n_cu);
@@ -628,7 +642,7 @@ int setmhd_upload_structured(setmhd_ctx* c, int32_t nx, int32_t ny, int32_t nz, 
   HIPCHK(c, hipStreamSynchronize(c->stream));
   hipFree(stage);
   c->sg = make_sgrid(c->cfg, nx, ny, nz, box);
-  c->kchunk = grid_kchunk(c, nx, ny, nz);
+  c->kchunk = grid_kchunk(c, nx, ny, nz, false);
   c->cur_buf = 0;
   c->n_cell = nc;
   c->n_gran = nz > 1 ? (long long)(nx + 1) * ny * nz + (long long)nx * (ny + 1) * nz +

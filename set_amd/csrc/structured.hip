@@ -599,90 +599,76 @@ __device__ __forceinline__ TileRed row_tile(const FusedArgs& a, int b, bool book
   double rs_d = 0.0, rs_r = 0.0;
   double accp[8];
   bool ok = true;
+  // Rows j0 - 1 (the row below the chunk: its y-side only) through j1 (the
+  // closing y-face above the chunk's last row, from row min(j1, ny - 1))
+  // take the same code as the owned rows, so the result does not depend on
+  // the chunking (the loop is not peeled: a peeled copy of the flux code
+  // may contract multiply-adds differently).
   Side<R> yp;
-  {  // y-face data of the row below the chunk
-    double U[8];
-    load8(a.Uf, g.nc, (unsigned)cc + row * (unsigned)max(j0 - 1, 0), U);
-    CellQ<R> q;
-    derive<R, false>(U, q);
-    yp = side_of(q, 1);
-  }
   double Unx[8];
-  load8(a.Uf, g.nc, (unsigned)cc + row * (unsigned)j0, Unx);
-
-  for (int j = j0; j < j1; ++j) {
+  load8(a.Uf, g.nc, (unsigned)cc + row * (unsigned)max(j0 - 1, 0), Unx);
+#pragma clang loop unroll(disable)
+  for (int j = j0 - 1; j <= j1; ++j) {
+    const bool below = j < j0, top = j == j1;
     const unsigned idx = (unsigned)cc + row * (unsigned)j;
     double Uc[8];
 #pragma unroll
     for (int v = 0; v < 8; v++) Uc[v] = Unx[v];
-    load8(a.Uf, g.nc, (unsigned)cc + row * (unsigned)min(j + 1, g.ny - 1), Unx);  // next row
+    if (!top) load8(a.Uf, g.nc, (unsigned)cc + row * (unsigned)min(j + 1, g.ny - 1), Unx);  // next row
     CellQ<R> qc;
     derive<R, false>(Uc, qc);
-    sm.q[XR][l] = qc.rho;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      sm.q[XU + k][l] = qc.u[k];
-      sm.q[XB + k][l] = qc.b[k];
-    }
-    sm.q[XPT][l] = qc.pt;
-    sm.q[XE][l] = qc.e;
-    sm.q[XCF][l] = qc.cf[0];
-    double Uprev[8];  // U^n of the cell below, updated once the y-face between them is known
-    load8(a.Uf, g.nc, j > j0 ? idx - row : idx, Uprev);
-    __syncthreads();
-
-    if (l < RW - 1) {  // x-face on the right of this lane's cell
-      R F[8], s;
-      ok &= hlld(side_of(qc, 0), row_side(sm, l + 1), F, s);
-      smax[0] = fmax(smax[0], (double)s);
-#pragma unroll
-      for (int v = 0; v < 8; v++) sm.fx[v][l] = F[v];
-    }
-    R Fy[8];
     const Side<R> yc = side_of(qc, 1);
-    {  // y-face below this lane's cell
-      R s;
-      ok &= hlld(yp, yc, Fy, s);
-      smax[1] = fmax(smax[1], (double)s);
-    }
-    __syncthreads();
-
-    // signed face sum in the face-slot order (x-, x+, y-, y+)
-    double fy[8];
-    from_y_frame(Fy, fy);
-    double acc[8];
-    const int lm = max(l - 1, 0);
+    if (!below) {
+      double Uprev[8];  // U^n of the cell below, updated once the y-face between them is known
+      if (!top) {
+        sm.q[XR][l] = qc.rho;
 #pragma unroll
-    for (int v = 0; v < 8; v++) {
-      double s = -((double)sm.fx[v][lm] * g.ax);
-      s = s + (double)sm.fx[v][l] * g.ax;
-      acc[v] = s - fy[v] * g.ay;
-    }
-    if (j > j0 && own) {
+        for (int k = 0; k < 3; k++) {
+          sm.q[XU + k][l] = qc.u[k];
+          sm.q[XB + k][l] = qc.b[k];
+        }
+        sm.q[XPT][l] = qc.pt;
+        sm.q[XE][l] = qc.e;
+        sm.q[XCF][l] = qc.cf[0];
+      }
+      load8(a.Uf, g.nc, j > j0 ? idx - row : idx, Uprev);
+      if (!top) {
+        __syncthreads();
+        if (l < RW - 1) {  // x-face on the right of this lane's cell
+          R F[8], s;
+          ok &= hlld(side_of(qc, 0), row_side(sm, l + 1), F, s);
+          smax[0] = fmax(smax[0], (double)s);
 #pragma unroll
-      for (int v = 0; v < 8; v++) accp[v] = accp[v] + fy[v] * g.ay;
-      const double ycen = __dadd_rn(g.y0, __dmul_rn((double)(j - 1) + 0.5, g.dy));
-      finalize_with<MODE>(a, dt, idx - row, held_cell(g, xc, ycen, 0.0), accp, Uprev, rs_d, rs_r);
-    }
-#pragma unroll
-    for (int v = 0; v < 8; v++) accp[v] = acc[v];
-    yp = yc;
-  }
-  {  // y-face above the chunk's last row (row min(j1, ny-1), prefetched)
-    CellQ<R> q;
-    derive<R, false>(Unx, q);
-    R Fy[8], s;
-    ok &= hlld(yp, side_of(q, 1), Fy, s);
-    smax[1] = fmax(smax[1], (double)s);
-    if (own) {
+          for (int v = 0; v < 8; v++) sm.fx[v][l] = F[v];
+        }
+      }
+      R Fy[8];
+      {  // y-face below this lane's cell
+        R s;
+        ok &= hlld(yp, yc, Fy, s);
+        smax[1] = fmax(smax[1], (double)s);
+      }
       double fy[8];
       from_y_frame(Fy, fy);
+      if (j > j0 && own) {
 #pragma unroll
-      for (int v = 0; v < 8; v++) accp[v] = accp[v] + fy[v] * g.ay;
-      const double ycen = __dadd_rn(g.y0, __dmul_rn((double)(j1 - 1) + 0.5, g.dy));
-      finalize<MODE>(a, dt, (unsigned)cc + row * (unsigned)(j1 - 1), held_cell(g, xc, ycen, 0.0), accp,
-                     rs_d, rs_r);
+        for (int v = 0; v < 8; v++) accp[v] = accp[v] + fy[v] * g.ay;
+        const double ycen = __dadd_rn(g.y0, __dmul_rn((double)(j - 1) + 0.5, g.dy));
+        finalize_with<MODE>(a, dt, idx - row, held_cell(g, xc, ycen, 0.0), accp, Uprev, rs_d, rs_r);
+      }
+      if (!top) {
+        __syncthreads();
+        // signed face sum in the face-slot order (x-, x+, y-, y+)
+        const int lm = max(l - 1, 0);
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+          double s = -((double)sm.fx[v][lm] * g.ax);
+          s = s + (double)sm.fx[v][l] * g.ax;
+          accp[v] = s - fy[v] * g.ay;
+        }
+      }
     }
+    yp = yc;
   }
   if (!ok) atomicOr(a.err, kErrRegion);
 

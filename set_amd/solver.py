@@ -285,9 +285,10 @@ class Context:
         self._check(self._L.setmhd_batch_download(self._h, int(slot), _pd(cp), ctypes.byref(t)))
         return cp, t.value
 
-    def batch_set_launch(self, tile=(32, 8), blocks_per_cu=2, dim=0):
+    def batch_set_launch(self, tile=(32, 8), blocks_per_cu=0, dim=0):
         """Fixes the batch launch of 2-D (dim 2), 3-D (dim 3) or all (0)
-        grids; blocks_per_cu 0: one block per tile."""
+        grids; blocks_per_cu 0: one block per tile.  The tile shape applies
+        to 3-D grids (2-D grids march row segments)."""
         self._check(self._L.setmhd_batch_set_launch(self._h, int(dim), int(tile[0]), int(tile[1]),
                                                     int(blocks_per_cu)))
 

@@ -46,7 +46,7 @@ def test_batch_equals_alone(precision, launch):
     gs = grids()
     with solver.Context(solver.cartesian_config(precision=precision)) as c:
         if launch:
-            c.batch_set_launch(*launch)  # else the defaults (2-D persistent, 3-D per tile)
+            c.batch_set_launch(*launch)  # else the defaults (one block per tile)
         slots = [c.batch_add(g, W) for g, W in gs]
         c.batch_step(37)
         res = [c.batch_download(s) for s in slots]
@@ -118,6 +118,7 @@ def test_autotune_restores_state():
         tuned = c.batch_autotune(probe_steps=3)
         for tx, ty, bpc in tuned.values():
             assert (tx, ty) in {(32, 8), (16, 16), (64, 4), (16, 8)} and bpc in (0, 1, 2, 4, 8, 64)
+        assert tuned[2][:2] == (32, 8)  # 2-D grids march rows whatever the tile
         assert all(c.batch_query(s)["steps"] == 4 for s in slots)
         c.batch_step(6)
         res = [c.batch_download(s)[0] for s in slots]

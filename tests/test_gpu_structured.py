@@ -29,9 +29,9 @@ def oracle_run(g, W, nsteps, time_scheme=0):
     return s, dt0
 
 
-def gpu_run(g, W, nsteps, precision=0, time_scheme=0, tile=(0, 0)):
+def gpu_run(g, W, nsteps, precision=0, time_scheme=0, tile=(0, 0), kchunk=0):
     with solver.Context(solver.cartesian_config(precision=precision, time_scheme=time_scheme,
-                                                tile=tile)) as c:
+                                                tile=tile, kchunk=kchunk)) as c:
         c.upload_structured(g, W)
         dt0 = c.dt_from_state()
         c.step(nsteps)
@@ -57,14 +57,35 @@ def test_cfg1_2d_parity(nsteps, bound, order):
     assert np.all(e <= bound), e
 
 
-@pytest.mark.parametrize("tile", [(32, 8), (16, 16), (64, 4), (16, 8)])
-def test_tiles_agree_bitwise(tile):
-    case(f"structured 2-D 130x70 tile={tile}")
-    g = mesh.grid_2d(130, 70)
+@pytest.mark.parametrize("kchunk", [1, 2, 5, 64])
+def test_row_chunks_agree_2d(kchunk):
+    """2-D grids march 254-column row segments up chunks of kchunk rows (600
+    columns: segments of 254, 254 and 92).  The row below a chunk and the
+    face above it take the owned rows' code, so any chunking gives the same
+    bits."""
+    case(f"structured 2-D 600x70 kchunk={kchunk} vs 16, steps=20")
+    g = mesh.grid_2d(600, 70)
     W = mesh.heliosphere_state(g)
-    ref, _, _ = gpu_run(g, W, 20, tile=(32, 8))
-    cp, _, _ = gpu_run(g, W, 20, tile=tile)
+    ref, tr, _ = gpu_run(g, W, 20, kchunk=16)
+    cp, t, _ = gpu_run(g, W, 20, kchunk=kchunk)
+    e = rel_err(cons(cp), cons(ref), "inf")
+    print("rel Linf per var", e)
+    assert t == tr
     np.testing.assert_array_equal(cp, ref)
+
+
+@pytest.mark.parametrize("tile", [(16, 16), (64, 4), (16, 8)])
+def test_3d_tiles_agree(tile):
+    """3-D tile shapes against 32x8 (ragged tiles in x and y), FP64."""
+    case(f"structured 3-D 70x36x20 tile={tile} vs 32x8, steps=10")
+    g = mesh.grid_3d(70, 36, 20)
+    W = mesh.heliosphere_state(g, wind=mesh.Wind(r_in=0.3))
+    ref, tr, _ = gpu_run(g, W, 10, tile=(32, 8))
+    cp, t, _ = gpu_run(g, W, 10, tile=tile)
+    e = rel_err(cons(cp), cons(ref), "inf")
+    print("rel Linf per var", e, "bitwise", np.array_equal(cp, ref))
+    assert t == tr
+    assert np.all(e <= 1e-12), e
 
 
 def test_3d_parity_one_step():
