@@ -605,7 +605,7 @@ __device__ __forceinline__ TileRed row_tile(const FusedArgs& a, int b, bool book
   // the chunking (the loop is not peeled: a peeled copy of the flux code
   // may contract multiply-adds differently).
   Side<R> yp;
-  double Unx[8];
+  double Unx[8], Uprev[8];  // Uprev: U^n of the cell below, updated once the y-face between them is known
   load8(a.Uf, g.nc, (unsigned)cc + row * (unsigned)max(j0 - 1, 0), Unx);
 #pragma clang loop unroll(disable)
   for (int j = j0 - 1; j <= j1; ++j) {
@@ -619,7 +619,6 @@ __device__ __forceinline__ TileRed row_tile(const FusedArgs& a, int b, bool book
     derive<R, false>(Uc, qc);
     const Side<R> yc = side_of(qc, 1);
     if (!below) {
-      double Uprev[8];  // U^n of the cell below, updated once the y-face between them is known
       if (!top) {
         sm.q[XR][l] = qc.rho;
 #pragma unroll
@@ -631,7 +630,6 @@ __device__ __forceinline__ TileRed row_tile(const FusedArgs& a, int b, bool book
         sm.q[XE][l] = qc.e;
         sm.q[XCF][l] = qc.cf[0];
       }
-      load8(a.Uf, g.nc, j > j0 ? idx - row : idx, Uprev);
       if (!top) {
         __syncthreads();
         if (l < RW - 1) {  // x-face on the right of this lane's cell
@@ -669,6 +667,8 @@ __device__ __forceinline__ TileRed row_tile(const FusedArgs& a, int b, bool book
       }
     }
     yp = yc;
+#pragma unroll
+    for (int v = 0; v < 8; v++) Uprev[v] = Uc[v];
   }
   if (!ok) atomicOr(a.err, kErrRegion);
 
