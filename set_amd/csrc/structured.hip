@@ -354,7 +354,9 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
     wait_vm();
     __syncthreads();
   }
-  double Unx[8], Hnx[8];
+  // Ukeep: U^n of this column's previous plane, kept in registers (a
+  // re-read would make its wait drain the next plane's prefetch as well)
+  double Unx[8], Hnx[8], Ukeep[8] = {};
   if constexpr (PF) {
     load8(a.Uf, g.nc, col0 + plane * k0, Unx);
     if (has_h) load8(a.Uf, g.nc, hcol + plane * k0, Hnx);
@@ -382,6 +384,12 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
       load8(a.Uf, g.nc, idx, Uc);
       if (has_h) load8(a.Uf, g.nc, hcol + plane * k, Uh);
     }
+    double Uprev[8];  // U^n of the cell below, updated once the z-face between them is known
+#pragma unroll
+    for (int v = 0; v < 8; v++) {
+      Uprev[v] = Ukeep[v];
+      Ukeep[v] = Uc[v];
+    }
     CellQ<R> qc;
     derive<R, DIM == 3>(Uc, qc);
     put_q(sm, ty + 1, tx + 1, qc);
@@ -390,8 +398,6 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
       derive<R, false>(Uh, qh);
       put_q(sm, hly, hlx, qh);
     }
-    double Uprev[8];  // U^n of the cell below, updated once the z-face between them is known
-    if (DIM == 3) load8(a.Uf, g.nc, k > k0 ? idx - plane : idx, Uprev);
     __syncthreads();
     // every wave has read the staged plane: stage plane k+1 (the epilogue's
     // plane when k+1 == k1)
