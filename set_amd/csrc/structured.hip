@@ -430,7 +430,10 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
       ok &= hlld(zp, zc, Fz, s);
       smax[2] = fmax(smax[2], (double)s);
     }
-    if constexpr (DMA) wait_vm();  // this wave's share of plane k+1 has landed
+    // DMA: this wave's share of plane k+1 has landed.  Otherwise the
+    // prefetch, waited for here rather than at the loop's back edge, where
+    // the wait would also drain the finalize stores issued after it.
+    wait_vm();
     __syncthreads();
 
     // signed face sum in the face-slot order (x-, x+, y-, y+, z-, z+)
