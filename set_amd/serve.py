@@ -20,6 +20,7 @@ run_sequential is the one-at-a-time baseline: each request is uploaded,
 solved alone with the single-grid path and downloaded before the next.
 """
 import collections
+import contextlib
 import time
 
 import numpy as np
@@ -44,114 +45,162 @@ def _latency_stats(done, wall):
     }
 
 
+class _Lane:
+    """One batch context and the requests routed to it."""
+
+    def __init__(self, ctx, pending):
+        self.ctx = ctx
+        self.pending = collections.deque(pending)  # request indices, arrival order
+        self.active = {}  # slot -> [request index, admitted time, steps done]
+        self.in_flight = False
+        self.active_cells = 0
+        self.tuned = False
+
+    def busy(self):
+        return bool(self.pending or self.active or self.in_flight)
+
+
 class Server:
     def __init__(self, precision=solver.PREC_FP64, check_every=50, max_steps=5000, tol=1e-6,
                  reserve_bytes=4 << 30, max_active=512, max_active_cells=None, autotune=False,
-                 context=None, poll_s=0.001):
+                 context=None, poll_s=0.001, split_dims=True):
         self.precision = precision
         self.check_every = check_every
         self.max_steps = max_steps
         self.tol = tol
         self.reserve = reserve_bytes
         self.max_active = max_active
-        # optional cap on the cells stepping at once (beyond what fills the
-        # GPU, more concurrency only stretches every request's latency);
-        # a request larger than the cap still runs, alone
+        # optional cap on the cells stepping at once per lane (beyond what
+        # fills the GPU, more concurrency only stretches every request's
+        # latency); a request larger than the cap still runs, alone
         self.max_active_cells = max_active_cells
-        # Off by default: the library's launches (2-D: persistent blocks;
-        # 3-D: one block per tile) measured better on a changing request mix
-        # than a tuning done on the first two requests.
+        # Off by default: the library's launches (one block per tile)
+        # measured better on a changing request mix than a tuning done on
+        # the first two requests.
         self.autotune = autotune
-        self.poll_s = poll_s  # host sleep while a chunk runs and requests are due
+        self.poll_s = poll_s  # host sleep while chunks run
+        # 2-D and 3-D requests in two batch contexts (two streams): the
+        # small 2-D grids advance at their own step rate instead of the
+        # large 3-D grids' (one context when False)
+        self.split_dims = split_dims
         # context factory (a solver.Context by default; tests pass a model)
         self.context = context or (lambda: solver.Context(solver.cartesian_config(precision=precision)))
+
+    def _admit(self, lane, requests, states, now, free, clock, t0, spent):
+        """Admits lane's arrived requests in arrival order while memory and
+        the caps allow; returns the free bytes left."""
+        while (lane.pending and requests[lane.pending[0]].arrival <= now
+               and len(lane.active) < self.max_active):
+            i = lane.pending[0]
+            need = requests[i].cells * BYTES_PER_CELL
+            if free - need < self.reserve:
+                break  # admitted when memory frees up
+            if (self.max_active_cells is not None and lane.active
+                    and lane.active_cells + requests[i].cells > self.max_active_cells):
+                break  # admitted when running requests retire
+            lane.pending.popleft()
+            ta = clock()
+            slot = lane.ctx.batch_add(requests[i].grid, states[i])
+            spent["admit"] += clock() - ta
+            lane.active[slot] = [i, clock() - t0, 0]
+            lane.active_cells += requests[i].cells
+            free -= need
+        return free
+
+    def _complete(self, lane, requests, spent, clock):
+        """Completes lane's chunk and removes its converged or capped
+        requests; returns them (readable until the lane's next wait)."""
+        ta = clock()
+        lane.ctx.batch_wait()
+        spent["step"] += clock() - ta
+        lane.in_flight = False
+        retiring = []
+        for slot in list(lane.active):
+            q = lane.ctx.batch_query(slot)
+            lane.active[slot][2] = q["steps"]
+            if q["steps"] == 0:
+                continue  # admitted while the chunk ran
+            conv = q["residual"] <= self.tol
+            if conv or q["steps"] >= self.max_steps:
+                lane.ctx.batch_remove(slot)
+                i, admitted, steps = lane.active.pop(slot)
+                lane.active_cells -= requests[i].cells
+                retiring.append((slot, i, admitted, steps, q["residual"], conv))
+        return retiring
+
+    def _launch(self, lane, spent, clock):
+        if not lane.tuned and len(lane.active) > 1:
+            ta = clock()
+            lane.ctx.batch_autotune(probe_steps=2)
+            spent["autotune"] += clock() - ta
+            lane.tuned = True
+        chunk = min(self.check_every, min(self.max_steps - a[2] for a in lane.active.values()))
+        lane.ctx.batch_step_async(chunk)
+        lane.in_flight = True
 
     def run(self, requests, states, clock=time.perf_counter, sleep=time.sleep, keep_results=False):
         """requests: workloads.Request list; states: their initial Cell_par
         (same order).  Returns per-request records and summary stats.
 
-        Each pass of the loop admits arrived requests (their uploads run on
-        the library's side stream, beside the chunk in flight), waits for
-        the chunk, retires converged or capped requests, starts the next
-        chunk, and downloads the retired results while it runs."""
+        Each pass of the loop, per lane: admits arrived requests (their
+        uploads run on the library's side stream, beside the chunk in
+        flight); when the lane's chunk has ended, retires converged or
+        capped requests, starts the next chunk, and downloads the retired
+        results while it runs."""
         order = sorted(range(len(requests)), key=lambda i: requests[i].arrival)
-        pending = collections.deque(order)
-        active = {}  # slot -> [request index, admitted time, steps done]
+        keys = [0, 1] if self.split_dims else [0]
+        route = (lambda i: int(requests[i].grid.nz > 1)) if self.split_dims else (lambda i: 0)
         done = []
-        tuned = not self.autotune
         spent = collections.Counter()  # seconds per server activity
-        active_cells = 0
-        with self.context() as ctx:
+        with contextlib.ExitStack() as stack:
+            lanes = []
+            for k in keys:
+                mine = [i for i in order if route(i) == k]
+                if mine:
+                    lanes.append(_Lane(stack.enter_context(self.context()), mine))
+            for lane in lanes:
+                lane.tuned = not self.autotune
             t0 = clock()
-            in_flight = False
-            while pending or active or in_flight:
-                now = clock() - t0
-                free, _ = ctx.mem_info()
-                while (pending and requests[pending[0]].arrival <= now
-                       and len(active) < self.max_active):
-                    i = pending[0]
-                    need = requests[i].cells * BYTES_PER_CELL
-                    if free - need < self.reserve:
-                        break  # admitted when memory frees up
-                    if (self.max_active_cells is not None and active
-                            and active_cells + requests[i].cells > self.max_active_cells):
-                        break  # admitted when running requests retire
-                    pending.popleft()
-                    ta = clock()
-                    slot = ctx.batch_add(requests[i].grid, states[i])
-                    spent["admit"] += clock() - ta
-                    active[slot] = [i, clock() - t0, 0]
-                    active_cells += requests[i].cells
-                    free -= need
-                retiring = []
-                if in_flight and pending and not ctx.batch_poll():
-                    # keep admitting arrivals while the chunk runs
-                    nxt = requests[pending[0]].arrival - (clock() - t0)
-                    sleep(min(nxt, self.poll_s) if nxt > 0.0 else self.poll_s)
-                    continue
-                if in_flight:
-                    ta = clock()
-                    ctx.batch_wait()
-                    spent["step"] += clock() - ta
-                    in_flight = False
-                    for slot in list(active):
-                        q = ctx.batch_query(slot)
-                        active[slot][2] = q["steps"]
-                        if q["steps"] == 0:
-                            continue  # admitted while the chunk ran
-                        conv = q["residual"] <= self.tol
-                        if conv or q["steps"] >= self.max_steps:
-                            ctx.batch_remove(slot)  # readable until the next wait
-                            i, admitted, steps = active.pop(slot)
-                            active_cells -= requests[i].cells
-                            retiring.append((slot, i, admitted, steps, q["residual"], conv))
-                if active:
-                    if not tuned and len(active) > 1:
+            while any(lane.busy() for lane in lanes):
+                free, _ = lanes[0].ctx.mem_info()
+                progressed = False
+                for lane in lanes:
+                    free = self._admit(lane, requests, states, clock() - t0, free, clock, t0, spent)
+                    retiring = []
+                    if lane.in_flight:
+                        # block only when nothing else can need the host
+                        alone = all(not o.busy() for o in lanes if o is not lane) and not lane.pending
+                        if not alone and not lane.ctx.batch_poll():
+                            continue
+                        retiring = self._complete(lane, requests, spent, clock)
+                        progressed = True
+                    if lane.active and not lane.in_flight:
+                        self._launch(lane, spent, clock)
+                        progressed = True
+                    for slot, i, admitted, steps, res, conv in retiring:
                         ta = clock()
-                        ctx.batch_autotune(probe_steps=2)
-                        spent["autotune"] += clock() - ta
-                        tuned = True
-                    chunk = min(self.check_every,
-                                min(self.max_steps - a[2] for a in active.values()))
-                    ctx.batch_step_async(chunk)
-                    in_flight = True
-                for slot, i, admitted, steps, res, conv in retiring:
-                    ta = clock()
-                    cp, t = ctx.batch_download(slot)  # overlaps the chunk just started
-                    spent["retire"] += clock() - ta
-                    done.append({"rid": requests[i].rid, "arrival": requests[i].arrival,
-                                 "admitted": admitted, "done": clock() - t0, "steps": steps,
-                                 "residual": res, "converged": conv, "cells": requests[i].cells,
-                                 "t": t, "result": cp if keep_results else None})
-                if not active and not in_flight and pending:
-                    wait = requests[pending[0]].arrival - (clock() - t0)
-                    if wait > 0.0:
-                        sleep(wait)
-                    elif retiring:
-                        ctx.batch_wait()  # frees the retired requests' memory; admit again
-                    else:  # arrived, nothing running, still not admitted
-                        raise RuntimeError(f"request {requests[pending[0]].rid} does not fit in "
-                                           "device memory")
+                        cp, t = lane.ctx.batch_download(slot)  # overlaps the chunk just started
+                        spent["retire"] += clock() - ta
+                        done.append({"rid": requests[i].rid, "arrival": requests[i].arrival,
+                                     "admitted": admitted, "done": clock() - t0, "steps": steps,
+                                     "residual": res, "converged": conv,
+                                     "cells": requests[i].cells, "t": t,
+                                     "result": cp if keep_results else None})
+                    if not lane.active and not lane.in_flight and retiring:
+                        lane.ctx.batch_wait()  # frees the retired requests' memory
+                if progressed:
+                    continue
+                now = clock() - t0
+                arrivals = [requests[lane.pending[0]].arrival for lane in lanes if lane.pending]
+                nxt = min(arrivals) - now if arrivals else None
+                if any(lane.in_flight for lane in lanes):
+                    sleep(min(nxt, self.poll_s) if nxt is not None and nxt > 0.0 else self.poll_s)
+                elif nxt is not None and nxt > 0.0:
+                    sleep(nxt)
+                else:  # arrived, nothing running, still not admitted
+                    lane = next(lane for lane in lanes if lane.pending)
+                    raise RuntimeError(f"request {requests[lane.pending[0]].rid} does not fit in "
+                                       "device memory")
             wall = clock() - t0
         stats = _latency_stats(done, wall)
         stats["seconds_in"] = dict(spent)

@@ -26,12 +26,14 @@ def small_requests(n=7, seed=3):
     return out
 
 
-def test_each_result_same_alone_and_in_the_mix():
-    print("[case] serving: batched vs one-at-a-time results", flush=True)
+@pytest.mark.parametrize("split_dims", [True, False])
+def test_each_result_same_alone_and_in_the_mix(split_dims):
+    print(f"[case] serving: batched (split_dims={split_dims}) vs one-at-a-time results", flush=True)
     reqs = small_requests()
     states = [r.state() for r in reqs]
     kw = dict(check_every=20, max_steps=60, tol=1e-300)
-    done_b, stats_b = serve.Server(autotune=True, **kw).run(reqs, states, keep_results=True)
+    done_b, stats_b = serve.Server(autotune=True, split_dims=split_dims, **kw).run(
+        reqs, states, keep_results=True)
     done_s, stats_s = serve.run_sequential(reqs, states, keep_results=True, **kw)
     assert stats_b["solves"] == stats_s["solves"] == len(reqs)
     by_b = {d["rid"]: d for d in done_b}

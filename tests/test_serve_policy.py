@@ -166,3 +166,41 @@ def test_late_arrival_waits_and_latency_stats():
     assert stats["latency_p50_s"] == pytest.approx(np.percentile(lat, 50))
     assert stats["latency_p99_s"] == pytest.approx(np.percentile(lat, 99))
     assert stats["solves_per_hour"] == pytest.approx(3600 * 2 / stats["wall_s"])
+
+
+def mixed_run(split):
+    """One large 3-D request and three small 2-D ones, all at t = 0; every
+    lane gets its own model context on the shared clock."""
+    big = workloads.Request(0, 0.0, mesh.grid_3d(100, 100, 100), mesh.Inflow())
+    small = [workloads.Request(i, 0.0, mesh.grid_2d(1000, 1), mesh.Inflow()) for i in (1, 2, 3)]
+    reqs = [big] + small
+    conv = {10 ** 6: 1000, 1000: 100}
+    clock = SimClock()
+    made = []
+
+    def factory():
+        made.append(ModelContext(clock, 1 << 40, conv))
+        return made[-1]
+
+    srv = serve.Server(check_every=50, max_steps=5000, autotune=False, reserve_bytes=0,
+                       context=factory, split_dims=split)
+    done, stats = srv.run(reqs, [None] * 4, clock=clock, sleep=clock.sleep)
+    return {r["rid"]: r for r in done}, stats, made
+
+
+def test_split_lanes_let_small_requests_finish_first():
+    one, _, made1 = mixed_run(False)
+    two, _, made2 = mixed_run(True)
+    assert len(made1) == 1 and len(made2) == 2
+    # the same steps for every request either way
+    assert {k: r["steps"] for k, r in one.items()} == {k: r["steps"] for k, r in two.items()}
+    assert all(r["converged"] for r in two.values())
+    # alone in their lane the 2-D requests no longer step at the 3-D grid's pace
+    for rid in (1, 2, 3):
+        assert two[rid]["done"] < 0.5 * one[rid]["done"]
+
+
+def test_split_lanes_route_by_dimensionality():
+    _, _, made = mixed_run(True)
+    cells = [sorted(n for e, n, _ in ctx.log if e == "add") for ctx in made]
+    assert sorted(cells) == [[1000, 1000, 1000], [10 ** 6]]
