@@ -164,8 +164,8 @@ int setmhd_residual(setmhd_ctx *ctx, double *res);
 
 /* ---- batches of structured grids (continuous batching) -------------------
  * A context can instead hold many independent structured grids ("slots").
- * All admitted grids advance together: one persistent fused launch per step
- * per dimensionality, captured in a HIP graph; each grid keeps its own
+ * All admitted grids advance together: one fused launch per step per
+ * dimensionality, captured in a HIP graph; each grid keeps its own
  * lagged dt, time, step count and density residual on the device.  Grids
  * may be admitted and retired between calls.  Forward Euler only.
  * No counterpart in the reference (its solver holds one mesh); this is the

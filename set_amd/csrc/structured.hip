@@ -166,10 +166,8 @@ __device__ __forceinline__ bool held_cell(const SGrid& g, double xc, double yc, 
 }
 
 // Applies the face sum acc to cell idx, whose state U^n is Uold.  The
-// z-march reads U^n of the plane below back from the input (an L2 hit: it
-// was read one plane earlier) at the start of the plane's face phase, so the
-// load is in flight while the faces are formed, rather than keeping it live
-// in registers across the march.
+// marches update the row or plane below the current one, whose U^n they keep
+// in registers from the previous iteration.
 template <int MODE>
 __device__ __forceinline__ void finalize_with(const FusedArgs& a, double dt, unsigned idx, bool held,
                                               const double acc[8], const double Uold[8],

@@ -3,9 +3,10 @@
 Requests arrive over time (a seeded Poisson stream); each is a structured
 grid with its own inflow and field.  The server keeps every admitted
 request's state resident on the GPU as one slot of a batch context
-(setmhd_batch_*): all active requests advance together, one persistent
-fused launch per step per dimensionality, replayed from a captured graph.
-Around chunks of steps it
+(setmhd_batch_*): all active requests of a lane advance together, one
+fused launch per step, replayed from a captured graph.  2-D and 3-D
+requests run in two lanes (two contexts, two streams).  Around chunks of
+steps it
 
   * admits arrived requests in arrival order while free HBM (hipMemGetInfo)
     covers their state plus a reserve (uploads overlap the running chunk),
