@@ -188,7 +188,10 @@ def test_structured_equals_dropin_on_the_same_grid(shape, nsteps, order):
         assert np.all(e[:5] <= 1e-10), e
 
 
-TINY = [(1, 1, 1), (3, 1, 1), (1, 5, 1), (2, 2, 2), (1, 1, 7), (2, 3, 4), (33, 9, 1)]
+# 2-D row segments own 254 columns: 254, 255 and 509 columns end a segment
+# exactly, leave one column to a second segment, and to a third
+TINY = [(1, 1, 1), (3, 1, 1), (1, 5, 1), (2, 2, 2), (1, 1, 7), (2, 3, 4), (33, 9, 1), (254, 3, 1),
+        (255, 2, 1), (509, 2, 1)]
 
 
 def vec_err(a, b):
@@ -205,8 +208,8 @@ def vec_err(a, b):
 @pytest.mark.parametrize("shape", TINY)
 def test_tiny_and_ragged_grids(shape):
     """Grids narrower than a tile in every direction (one cell wide, one
-    tile plus one cell): FP64 against the oracle after 3 steps, and the
-    mixed form runs."""
+    tile plus one cell) and 2-D grids at row-segment boundaries: FP64
+    against the oracle after 3 steps, and the mixed form runs."""
     case(f"structured tiny grid {shape}")
     nx, ny, nz = shape
     g = mesh.grid_2d(nx, ny) if nz == 1 else mesh.grid_3d(nx, ny, nz)
