@@ -1,6 +1,6 @@
 """Throughput of the fused ideal-MHD step on one MI355X.
 
-Workload (BASELINE "plain method at scale"): the heliosphere set-up on a
+Workload: the heliosphere set-up on a
 3-D 256^3 box [-1,1]^3 -- supersonic radial wind held inside r < 0.1, a
 uniform magnetised inflow (rho 0.1 with +-1 % seeded noise, u = -2.5,
 p = 0.1, |B| = 0.3 tilted 30 deg in x-y) elsewhere -- first-order HLLD,
@@ -11,7 +11,9 @@ cell-updates/s over the timed steps (wall clock between device syncs, state
 resident in HBM).  The roofline object uses the kernel's own duration from
 HIP events on the library's stream and 128 algorithmic bytes per
 cell-update (read U^n, write U^{n+1}: 8 FP64 each).  The mixed-precision
-and Heun-RK2 forms are reported beside it under "forms".  cpu_baseline is
+and Heun-RK2 forms, the 64-grid batch, the serving stream and the drop-in
+face-list path at reference semantics are reported beside it under
+"forms".  cpu_baseline is
 the CPU oracle (the C restatement of the reference algorithm) on a bounded
 sample of the same workload.
 
@@ -129,6 +131,28 @@ def serving_form(n_requests, max_steps, precision=0):
             "solves_per_hour_ratio": b["solves_per_hour"] / q["solves_per_hour"]}
 
 
+def dropin_form(nx, ny, steps):
+    """The drop-in face-list path at SET's production semantics (HLLC with the
+    LAX wedge, minmod TVD, Heun, axisymmetric and charge-exchange sources,
+    w = B = 0, reference initial dt 2e-9) on a half-plane heliosphere mesh:
+    the exact build (bit for bit with the oracle) and the fast build."""
+    from set_amd import solver, workloads
+    fm, W = workloads.heliosphere_face_mesh(nx, ny)
+    out = {"mesh": f"{nx}x{ny} half-plane heliosphere face list", "cells": fm.n_cell,
+           "faces": fm.n_gran, "steps": steps, "time_scheme": "heun"}
+    for name, flags in (("exact", 0), ("fast", solver.F_FAST)):
+        cfg = solver.reference_config()
+        cfg.flags |= flags
+        with solver.Context(cfg) as c:
+            c.upload_mesh(fm, W)
+            c.set_dt(2.0e-9)
+            c.step(3)
+            c.sync()
+            ms = c.step_timed(steps) / steps
+        out[name] = {"value": fm.n_cell / (ms * 1e-3), "unit": "cell-updates/s", "ms_per_step": ms}
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -175,13 +199,15 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--forms", default="fp64,mixed,heun,batch64,serving")
+    ap.add_argument("--forms", default="fp64,mixed,heun,batch64,serving,dropin")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-max-steps", type=int, default=5000)
+    ap.add_argument("--dropin-mesh", default="2000x1000")
+    ap.add_argument("--dropin-steps", type=int, default=20)
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -218,7 +244,9 @@ def main():
     del W
     for name, fn in (("batch64_fp64", lambda: batch_form(0, args.batch_steps, 20)),
                      ("batch64_mixed", lambda: batch_form(1, args.batch_steps, 20)),
-                     ("serving_fp64", lambda: serving_form(args.serve_requests, args.serve_max_steps))):
+                     ("serving_fp64", lambda: serving_form(args.serve_requests, args.serve_max_steps)),
+                     ("dropin_reference", lambda: dropin_form(*map(int, args.dropin_mesh.split("x")),
+                                                              args.dropin_steps))):
         if name.split("_")[0] in wanted:
             try:
                 forms[name] = fn()

@@ -6,6 +6,8 @@
 * serving_requests: a Poisson stream of solves (mean rate 20/s), 60 % 2-D
   (256x128 to 1024x512) and 40 % 3-D (64^3 to 192^3), the same parameter
   ranges.
+* heliosphere_face_mesh: a half-plane heliosphere on SET's face-list arrays
+  for the drop-in path at reference semantics.
 """
 from dataclasses import dataclass
 
@@ -59,3 +61,22 @@ def serving_requests(n, rate=20.0, seed=7, frac_2d=0.6):
             g = mesh.grid_3d(m, m, m)
         out.append(Request(i, t, g, _params(rng)))
     return out
+
+
+def heliosphere_face_mesh(nx, ny, seed=5):
+    """Half-plane heliosphere mesh as the reference uses it: y = 0 is the
+    symmetry axis (Gran_type 5), the other sides are outer boundaries, the
+    field is zero (w = B = 0 semantics) and the neutral-H moments
+    Cell_par(9:16) are seeded positive values.  Returns (FaceMesh, Cell_par)."""
+    g = mesh.grid_2d(nx, ny)
+    fm = mesh.face_mesh(g)
+    axis = (fm.gran_normal[:, 1] == -1.0) & (fm.gran_center[:, 1] == g.y0)
+    fm.gran_type[axis] = mesh.TYPE_AXIS
+    W = mesh.heliosphere_state(g, inflow=mesh.Inflow(b=0.0))
+    rng = np.random.default_rng(seed)
+    for base in (8, 12):
+        W[:, base] = rng.uniform(0.05, 0.2, fm.n_cell)
+        W[:, base + 1] = rng.uniform(-3.0, -2.0, fm.n_cell)
+        W[:, base + 2] = rng.uniform(-0.2, 0.2, fm.n_cell)
+        W[:, base + 3] = rng.uniform(0.5, 1.5, fm.n_cell)
+    return fm, W

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
-from set_amd import mesh, solver
+from set_amd import mesh, solver, workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -77,22 +77,7 @@ def test_3d_bitwise():
     np.testing.assert_array_equal(cp[:, :8], s.W[:, :8])
 
 
-def helio_mesh(nx, ny, seed=5):
-    """Half-plane heliosphere mesh as the reference uses it: y = 0 is the
-    symmetry axis (Gran_type 5), the other sides are outer boundaries; the
-    neutral-H moments are seeded positive values."""
-    g = mesh.grid_2d(nx, ny)
-    fm = mesh.face_mesh(g)
-    axis = (fm.gran_normal[:, 1] == -1.0) & (fm.gran_center[:, 1] == g.y0)
-    fm.gran_type[axis] = mesh.TYPE_AXIS
-    W = mesh.heliosphere_state(g, inflow=mesh.Inflow(b=0.0))
-    rng = np.random.default_rng(seed)
-    for base in (8, 12):
-        W[:, base] = rng.uniform(0.05, 0.2, fm.n_cell)
-        W[:, base + 1] = rng.uniform(-3.0, -2.0, fm.n_cell)
-        W[:, base + 2] = rng.uniform(-0.2, 0.2, fm.n_cell)
-        W[:, base + 3] = rng.uniform(0.5, 1.5, fm.n_cell)
-    return fm, W
+helio_mesh = workloads.heliosphere_face_mesh  # half-plane heliosphere, y = 0 the axis
 
 
 def test_reference_semantics():
