@@ -300,25 +300,28 @@ UCfg ucfg(const setmhd_config& k) {
   return UCfg{k.riemann, k.recon, k.flags, k.cfl, k.dt_cap, k.r_skip_face, k.r_hold_cell};
 }
 
-// planes (3-D) or rows (2-D) one block marches, the configured value if set.
+// planes (3-D) or rows (2-D) one tile marches, the configured value if set.
 // 3-D: enough blocks for about four per CU with 32x8 columns, within [8, 32]
 // planes (each chunk re-reads two boundary planes).  2-D in a batch: 16
-// rows (the grids fill the chip together).  A 2-D grid alone: a block
+// rows (the grids fill the chip together).  A 2-D grid alone: a tile
 // re-reads one row below its chunk and starts up in about two row-times,
-// and the launch runs in rounds of two resident blocks per CU (the FP64
-// occupancy; a third mixed-form block adds little), so take the chunk in
+// and the launch runs in rounds of the resident tiles (two blocks per CU,
+// the FP64 occupancy; a third mixed-form block adds little; a block is one
+// tile of block rows or WPB tiles of wave rows), so take the chunk in
 // [1, 32] with the fewest row-times over its rounds, (k + 3) x rounds,
-// among those whose blocks fill 3/4 of a round (one row when none does).
+// among those whose tiles fill half a round (one row when none does).
 int grid_kchunk(const setmhd_ctx* c, int nx, int ny, int nz, bool batch) {
   if (c->cfg.kchunk > 0) return c->cfg.kchunk;
   if (nz <= 1) {
     if (batch) return 16;
-    const long long nseg = (nx + ROWN - 1) / ROWN, slots = 2LL * c->n_cu;
+    const bool waves = nx <= kWaveRowsMaxNx;
+    const long long nseg = waves ? (nx + WRN - 1) / WRN : (nx + ROWN - 1) / ROWN;
+    const long long slots = (waves ? 2LL * WPB : 2LL) * c->n_cu;
     int best = 1;
     long long best_cost = -1;
     for (int k = 1; k <= 32; k++) {
       const long long nb = nseg * ((ny + k - 1) / k);
-      if (4 * nb < 3 * slots) break;  // nb does not grow with k
+      if (2 * nb < slots) break;  // nb does not grow with k
       const long long cost = (nb + slots - 1) / slots * (k + 3);
       if (best_cost < 0 || cost <= best_cost) {
         best = k;

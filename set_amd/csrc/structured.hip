@@ -727,6 +727,157 @@ __global__ __launch_bounds__(RW) void k_rows(FusedArgs a) {
   }
 }
 
+// Wave row march: every wave marches its own WRN = 62-column segment (lanes
+// 1..62 own, 0 and 63 are the halo) up its own chunk of rows, with the same
+// arithmetic in the same order as the block row march.  The neighbour's
+// x-face data and the left x-face's flux cross lanes by shuffles, so the
+// march has no barrier and no LDS, and narrow grids waste fewer lanes.
+
+template <class R>
+__device__ __forceinline__ Side<R> shfl_down1(Side<R> s) {
+  s.rho = __shfl_down(s.rho, 1, 64);
+  s.vn = __shfl_down(s.vn, 1, 64);
+  s.vt1 = __shfl_down(s.vt1, 1, 64);
+  s.vt2 = __shfl_down(s.vt2, 1, 64);
+  s.bn = __shfl_down(s.bn, 1, 64);
+  s.bt1 = __shfl_down(s.bt1, 1, 64);
+  s.bt2 = __shfl_down(s.bt2, 1, 64);
+  s.pt = __shfl_down(s.pt, 1, 64);
+  s.e = __shfl_down(s.e, 1, 64);
+  s.cf = __shfl_down(s.cf, 1, 64);
+  return s;
+}
+
+// One wave tile t (segment t % nseg, row chunk t / nseg) of one grid; an
+// inactive wave marches nothing.  Returns the tile's CFL candidate and
+// density-residual sums, reduced over the wave (valid in every lane).
+template <class R, int MODE>
+__device__ __forceinline__ TileRed wrow_march(const FusedArgs& a, int t, bool active, bool book) {
+  const SGrid& g = a.g;
+  const int l = threadIdx.x, lane = l & 63;
+  const int ntx = (g.nx + WRN - 1) / WRN;
+  const int seg = t % ntx, rc = t / ntx;
+  const int c = seg * WRN + lane - 1;
+  const int cc = min(max(c, 0), g.nx - 1);
+  const bool own = lane >= 1 && lane <= WRN && c < g.nx;
+  const int j0 = rc * a.kchunk, j1 = active ? min(j0 + a.kchunk, g.ny) : j0 - 2;
+  const unsigned row = (unsigned)g.nx;
+  const double dt = a.dt.slot[a.cur];
+  if (MODE != kModeHeun2 && book && l == 0) {
+    a.dt.t[0] = a.dt.t[0] + dt;                 // cuf_kernel.cuf:792-794
+    a.dt.slot[(a.cur + 2) % 3] = kDtReset;      // :790, for step n+2
+  }
+  const double xc = __dadd_rn(g.x0, __dmul_rn((double)cc + 0.5, g.dx));
+  double smax[2] = {0.0, 0.0};
+  double rs_d = 0.0, rs_r = 0.0;
+  double accp[8];
+  bool ok = true;
+  Side<R> yp;
+  double Unx[8], Uprev[8];
+  if (active) load8(a.Uf, g.nc, (unsigned)cc + row * (unsigned)max(j0 - 1, 0), Unx);
+#pragma clang loop unroll(disable)
+  for (int j = j0 - 1; j <= j1; ++j) {
+    const bool below = j < j0, top = j == j1;
+    const unsigned idx = (unsigned)cc + row * (unsigned)j;
+    double Uc[8];
+#pragma unroll
+    for (int v = 0; v < 8; v++) Uc[v] = Unx[v];
+    if (!top) load8(a.Uf, g.nc, (unsigned)cc + row * (unsigned)min(j + 1, g.ny - 1), Unx);  // next row
+    CellQ<R> qc;
+    derive<R, false>(Uc, qc);
+    const Side<R> yc = side_of(qc, 1);
+    if (!below) {
+      R Fx[8];
+      if (!top) {  // x-face on the right of this lane's cell (lane 63's is not used)
+        const Side<R> xs = side_of(qc, 0);
+        const Side<R> xr = shfl_down1(xs);
+        R s;
+        const bool okx = hlld(xs, xr, Fx, s);
+        if (lane < 63) {
+          ok &= okx;
+          smax[0] = fmax(smax[0], (double)s);
+        }
+      }
+      R Fy[8];
+      {  // y-face below this lane's cell
+        R s;
+        ok &= hlld(yp, yc, Fy, s);
+        smax[1] = fmax(smax[1], (double)s);
+      }
+      double fy[8];
+      from_y_frame(Fy, fy);
+      if (j > j0 && own) {
+#pragma unroll
+        for (int v = 0; v < 8; v++) accp[v] = accp[v] + fy[v] * g.ay;
+        const double ycen = __dadd_rn(g.y0, __dmul_rn((double)(j - 1) + 0.5, g.dy));
+        finalize_with<MODE>(a, dt, idx - row, held_cell(g, xc, ycen, 0.0), accp, Uprev, rs_d, rs_r);
+      }
+      if (!top) {
+        // signed face sum in the face-slot order (x-, x+, y-, y+); the x-face
+        // on the left comes from lane - 1 (the x frame is the global order)
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+          const R left = __shfl_up(Fx[v], 1, 64);
+          double s = -((double)left * g.ax);
+          s = s + (double)Fx[v] * g.ax;
+          accp[v] = s - fy[v] * g.ay;
+        }
+      }
+    }
+    yp = yc;
+#pragma unroll
+    for (int v = 0; v < 8; v++) Uprev[v] = Uc[v];
+  }
+  if (!ok) atomicOr(a.err, kErrRegion);
+
+  const bool want_rs = a.resid != nullptr && MODE != kModeHeun1;
+  double cand = kDtReset;
+  if (MODE != kModeHeun2 && active) {
+    cand = g.cfl * g.distx / smax[0];
+    cand = fmin(cand, g.cfl * g.disty / smax[1]);
+    cand = fmin(cand, g.dt_cap);
+  }
+  cand = wave_min(cand);
+  if (want_rs) {
+    rs_d = wave_sum(rs_d);
+    rs_r = wave_sum(rs_r);
+  }
+  return TileRed{cand, rs_d, rs_r};
+}
+
+// One 2-D grid, wave row march: block b runs wave tiles WPB b .. WPB b + WPB - 1
+// (XCD-contiguous block ranges).
+template <class R, int MODE>
+__global__ __launch_bounds__(RW) void k_wrows(FusedArgs a) {
+  __shared__ double red[3 * WPB];
+  int b = blockIdx.x;
+  const int nblk = gridDim.x;
+  if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int t = b * WPB + wid;
+  const bool active = t < wave_row_tiles(a.g, a.kchunk);
+  const TileRed w = wrow_march<R, MODE>(a, active ? t : 0, active, blockIdx.x == 0);
+  if ((threadIdx.x & 63) == 0) {
+    red[wid] = w.dt;
+    red[WPB + wid] = w.rd;
+    red[2 * WPB + wid] = w.rr;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    TileRed r{red[0], red[WPB], red[2 * WPB]};
+    for (int v = 1; v < WPB; v++) {
+      r.dt = fmin(r.dt, red[v]);
+      r.rd += red[WPB + v];
+      r.rr += red[2 * WPB + v];
+    }
+    if (MODE != kModeHeun2) atomic_min_pos(&a.dt.slot[(a.cur + 1) % 3], r.dt);
+    if (a.resid != nullptr && MODE != kModeHeun1) {
+      atomicAdd(&a.resid[0], r.rd);
+      atomicAdd(&a.resid[1], r.rr);
+    }
+  }
+}
+
 // One grid: one block per tile, tiles remapped so each XCD (blockIdx % 8)
 // gets a contiguous range (neighbouring tiles share halo lines in its L2).
 template <class R, int TX, int TY, int DIM, int MODE, bool DMA>
@@ -752,90 +903,9 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(R
   }
 }
 
-// Shared memory and tile routine of a batch block: the row march for 2-D
-// grids (launched with RW = 32 x 8 threads), the z-march for 3-D grids.
-template <class R, int TX, int TY, int DIM>
-using BlockSmem = std::conditional_t<DIM == 2, RowSmem<R>, Smem<R, TX, TY>>;
-
-template <class R, int TX, int TY, int DIM>
-__device__ __forceinline__ TileRed batch_tile(const FusedArgs& a, int b, BlockSmem<R, TX, TY, DIM>& sm,
-                                              double* red) {
-  if constexpr (DIM == 2) {
-    static_assert(TX * TY == RW, "2-D batch blocks are row segments");
-    return row_tile<R, kModeEuler>(a, b, false, sm, red);
-  } else {
-    return fused_tile<R, TX, TY, DIM, kModeEuler>(a, b, false, sm, red);
-  }
-}
-
-// Many grids, one persistent launch per step.  Block b walks a contiguous
-// range of the global tile list, cut so that every range carries about the
-// same number of planes (ranges in block order: dealing them out per XCD
-// measured 25 % slower for this persistent launch).  Each grid's step index lives on the
-// device (BatchGrid::nstep): it selects the dt slot and the ping-pong
-// buffer, so the launch has no per-step arguments and many steps can be
-// captured in one graph.  k_batch_book, launched after it on the same
-// stream, closes the step of every grid.
-template <class R, int TX, int TY, int DIM>
-__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 2))) void k_fused_batch(
-    const BatchGrid* __restrict__ grids, const int* __restrict__ tile_grid,
-    const int* __restrict__ tile_first, const int* __restrict__ blk_first) {
-  __shared__ BlockSmem<R, TX, TY, DIM> sm;
-  __shared__ double red[3 * TX * TY / 64];
-  // tables on the device: the batch can change between graph replays
-  const int t0 = blk_first[blockIdx.x], t1 = blk_first[blockIdx.x + 1];
-  // thread 0 folds the block's results over consecutive tiles of one grid
-  // and publishes them (one atomic each) when the grid changes
-  int cur = -1;
-  TileRed acc{kDtReset, 0.0, 0.0};
-  auto flush = [&]() {
-    const BatchGrid& fg = grids[cur];
-    atomic_min_pos(&fg.dt.slot[(*fg.nstep + 1) % 3], acc.dt);
-    atomicAdd(&fg.resid[0], acc.rd);
-    atomicAdd(&fg.resid[1], acc.rr);
-    acc = TileRed{kDtReset, 0.0, 0.0};
-  };
-  for (int t = t0; t < t1; t++) {
-    const int gi = tile_grid[t];
-    if (threadIdx.x == 0 && gi != cur && cur >= 0) flush();
-    cur = gi;
-    const BatchGrid& bg = grids[gi];
-    const int n = *bg.nstep;
-    FusedArgs a;
-    a.g = bg.g;
-    a.Uf = bg.buf[n & 1];
-    a.Un = a.Uf;
-    a.Uo = bg.buf[(n & 1) ^ 1];
-    a.dt = bg.dt;
-    a.cur = n % 3;
-    a.kchunk = bg.kchunk;
-    a.resid = bg.resid;
-    a.err = bg.err;
-    const TileRed r = batch_tile<R, TX, TY, DIM>(a, t - tile_first[gi], sm, red);
-    if (threadIdx.x == 0) {
-      acc.dt = fmin(acc.dt, r.dt);
-      acc.rd += r.rd;
-      acc.rr += r.rr;
-    }
-    __syncthreads();  // LDS is reused by the next tile
-  }
-  if (threadIdx.x == 0 && cur >= 0) flush();
-}
-
-// Many grids, one block per tile (the other batch launch shape): the
-// hardware balances tiles of unequal cost as blocks finish, as for a single
-// grid, with the same XCD-contiguous tile ranges; the launch size is the
-// tile count, so the captured graph is re-made when that changes.
-template <class R, int TX, int TY, int DIM>
-__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 2))) void k_fused_tiles(
-    const BatchGrid* __restrict__ grids, const int* __restrict__ tile_grid,
-    const int* __restrict__ tile_first) {
-  __shared__ BlockSmem<R, TX, TY, DIM> sm;
-  __shared__ double red[3 * TX * TY / 64];
-  int t = blockIdx.x;
-  const int nblk = gridDim.x;
-  if ((nblk & 7) == 0) t = (t & 7) * (nblk >> 3) + (t >> 3);
-  const int gi = tile_grid[t];
+// The launch arguments of grid gi of a batch at its current step: its step
+// index on the device selects the dt slot and the ping-pong buffer.
+__device__ __forceinline__ FusedArgs batch_args(const BatchGrid* __restrict__ grids, int gi) {
   const BatchGrid& bg = grids[gi];
   const int n = *bg.nstep;
   FusedArgs a;
@@ -848,12 +918,197 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(R
   a.kchunk = bg.kchunk;
   a.resid = bg.resid;
   a.err = bg.err;
-  const TileRed r = batch_tile<R, TX, TY, DIM>(a, t - tile_first[gi], sm, red);
+  return a;
+}
+
+// Publishes one grid's folded CFL candidate and residual sums.
+__device__ __forceinline__ void batch_publish(const BatchGrid& g, const TileRed& r) {
+  atomic_min_pos(&g.dt.slot[(*g.nstep + 1) % 3], r.dt);
+  atomicAdd(&g.resid[0], r.rd);
+  atomicAdd(&g.resid[1], r.rr);
+}
+
+// 2-D grids of a batch: wave tiles.  Each wave of the block marches global
+// tile t0 + wave (when below t1) of whatever grid it belongs to; the waves'
+// results go through LDS and thread 0 folds them, in tile order, into acc
+// for grid *cur, publishing acc whenever the grid changes.
+template <class R>
+__device__ __forceinline__ void wave_tiles_step(const BatchGrid* __restrict__ grids,
+                                                const int* __restrict__ tile_grid,
+                                                const int* __restrict__ tile_first, int t0, int t1,
+                                                double* red, int* gid, TileRed& acc, int& cur) {
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int t = t0 + wid;
+  const bool active = t < t1;
+  const int gi = active ? tile_grid[t] : -1;
+  TileRed w{kDtReset, 0.0, 0.0};
+  if (active) {
+    const FusedArgs a = batch_args(grids, gi);
+    w = wrow_march<R, kModeEuler>(a, t - tile_first[gi], true, false);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    gid[wid] = gi;
+    red[wid] = w.dt;
+    red[WPB + wid] = w.rd;
+    red[2 * WPB + wid] = w.rr;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int v = 0; v < WPB && gid[v] >= 0; v++) {
+      if (gid[v] != cur) {
+        if (cur >= 0) batch_publish(grids[cur], acc);
+        cur = gid[v];
+        acc = TileRed{kDtReset, 0.0, 0.0};
+      }
+      acc.dt = fmin(acc.dt, red[v]);
+      acc.rd += red[WPB + v];
+      acc.rr += red[2 * WPB + v];
+    }
+  }
+  __syncthreads();  // red and gid are reused
+}
+
+// Many grids, one persistent launch per step.  Block b walks a contiguous
+// range of the global tile list, cut so that every range carries about the
+// same number of planes or rows (ranges in block order: dealing them out per
+// XCD measured 25 % slower for this persistent launch).  Each grid's step
+// index lives on the device (BatchGrid::nstep): it selects the dt slot and
+// the ping-pong buffer, so the launch has no per-step arguments and many
+// steps can be captured in one graph.  k_batch_book, launched after it on
+// the same stream, closes the step of every grid.  3-D grids: z-march tiles.
+template <class R, int TX, int TY, int DIM>
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 2))) void k_fused_batch(
+    const BatchGrid* __restrict__ grids, const int* __restrict__ tile_grid,
+    const int* __restrict__ tile_first, const int* __restrict__ blk_first) {
+  static_assert(DIM == 3, "2-D grids of a batch march wave rows (k_wrow_batch)");
+  __shared__ Smem<R, TX, TY> sm;
+  __shared__ double red[3 * TX * TY / 64];
+  // tables on the device: the batch can change between graph replays
+  const int t0 = blk_first[blockIdx.x], t1 = blk_first[blockIdx.x + 1];
+  // thread 0 folds the block's results over consecutive tiles of one grid
+  // and publishes them (one atomic each) when the grid changes
+  int cur = -1;
+  TileRed acc{kDtReset, 0.0, 0.0};
+  for (int t = t0; t < t1; t++) {
+    const int gi = tile_grid[t];
+    if (threadIdx.x == 0 && gi != cur && cur >= 0) {
+      batch_publish(grids[cur], acc);
+      acc = TileRed{kDtReset, 0.0, 0.0};
+    }
+    cur = gi;
+    const FusedArgs a = batch_args(grids, gi);
+    const TileRed r = fused_tile<R, TX, TY, DIM, kModeEuler>(a, t - tile_first[gi], false, sm, red);
+    if (threadIdx.x == 0) {
+      acc.dt = fmin(acc.dt, r.dt);
+      acc.rd += r.rd;
+      acc.rr += r.rr;
+    }
+    __syncthreads();  // LDS is reused by the next tile
+  }
+  if (threadIdx.x == 0 && cur >= 0) batch_publish(grids[cur], acc);
+}
+
+// Many grids, one block per tile (the other batch launch shape): the
+// hardware balances tiles of unequal cost as blocks finish, as for a single
+// grid, with the same XCD-contiguous tile ranges; the launch size is the
+// tile count, so the captured graph is re-made when that changes.  3-D grids.
+template <class R, int TX, int TY, int DIM>
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 2))) void k_fused_tiles(
+    const BatchGrid* __restrict__ grids, const int* __restrict__ tile_grid,
+    const int* __restrict__ tile_first) {
+  static_assert(DIM == 3, "2-D grids of a batch march wave rows (k_wrow_tiles)");
+  __shared__ Smem<R, TX, TY> sm;
+  __shared__ double red[3 * TX * TY / 64];
+  int t = blockIdx.x;
+  const int nblk = gridDim.x;
+  if ((nblk & 7) == 0) t = (t & 7) * (nblk >> 3) + (t >> 3);
+  const int gi = tile_grid[t];
+  const FusedArgs a = batch_args(grids, gi);
+  const TileRed r = fused_tile<R, TX, TY, DIM, kModeEuler>(a, t - tile_first[gi], false, sm, red);
   if (threadIdx.x == 0) {
     atomic_min_pos(&a.dt.slot[(a.cur + 1) % 3], r.dt);
     atomicAdd(&a.resid[0], r.rd);
     atomicAdd(&a.resid[1], r.rr);
   }
+}
+
+// The 2-D grids of a batch in block rows (the mixed form), one block per
+// tile, XCD-contiguous tile ranges.
+template <class R>
+__global__ __launch_bounds__(RW) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 2))) void k_brow_tiles(
+    const BatchGrid* __restrict__ grids, const int* __restrict__ tile_grid,
+    const int* __restrict__ tile_first) {
+  __shared__ RowSmem<R> sm;
+  __shared__ double red[3 * WPB];
+  int t = blockIdx.x;
+  const int nblk = gridDim.x;
+  if ((nblk & 7) == 0) t = (t & 7) * (nblk >> 3) + (t >> 3);
+  const int gi = tile_grid[t];
+  const TileRed r = row_tile<R, kModeEuler>(batch_args(grids, gi), t - tile_first[gi], false, sm, red);
+  if (threadIdx.x == 0) batch_publish(grids[gi], r);
+}
+
+// The 2-D grids of a batch in block rows, persistent blocks: block b walks
+// its range of tiles, folding consecutive tiles of one grid.
+template <class R>
+__global__ __launch_bounds__(RW) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 2))) void k_brow_batch(
+    const BatchGrid* __restrict__ grids, const int* __restrict__ tile_grid,
+    const int* __restrict__ tile_first, const int* __restrict__ blk_first) {
+  __shared__ RowSmem<R> sm;
+  __shared__ double red[3 * WPB];
+  const int t0 = blk_first[blockIdx.x], t1 = blk_first[blockIdx.x + 1];
+  int cur = -1;
+  TileRed acc{kDtReset, 0.0, 0.0};
+  for (int t = t0; t < t1; t++) {
+    const int gi = tile_grid[t];
+    if (threadIdx.x == 0 && gi != cur && cur >= 0) {
+      batch_publish(grids[cur], acc);
+      acc = TileRed{kDtReset, 0.0, 0.0};
+    }
+    cur = gi;
+    const TileRed r = row_tile<R, kModeEuler>(batch_args(grids, gi), t - tile_first[gi], false, sm, red);
+    if (threadIdx.x == 0) {
+      acc.dt = fmin(acc.dt, r.dt);
+      acc.rd += r.rd;
+      acc.rr += r.rr;
+    }
+    __syncthreads();  // LDS is reused by the next tile
+  }
+  if (threadIdx.x == 0 && cur >= 0) batch_publish(grids[cur], acc);
+}
+
+// The 2-D grids of a batch, persistent blocks: block b walks its range of
+// wave tiles WPB at a time.
+template <class R>
+__global__ __launch_bounds__(RW) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 2))) void k_wrow_batch(
+    const BatchGrid* __restrict__ grids, const int* __restrict__ tile_grid,
+    const int* __restrict__ tile_first, const int* __restrict__ blk_first) {
+  __shared__ double red[3 * WPB];
+  __shared__ int gid[WPB];
+  const int t0 = blk_first[blockIdx.x], t1 = blk_first[blockIdx.x + 1];
+  int cur = -1;
+  TileRed acc{kDtReset, 0.0, 0.0};
+  for (int t = t0; t < t1; t += WPB)
+    wave_tiles_step<R>(grids, tile_grid, tile_first, t, t1, red, gid, acc, cur);
+  if (threadIdx.x == 0 && cur >= 0) batch_publish(grids[cur], acc);
+}
+
+// The 2-D grids of a batch, one block per WPB consecutive wave tiles
+// (XCD-contiguous block ranges); counts[0] is the tile count.
+template <class R>
+__global__ __launch_bounds__(RW) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 2))) void k_wrow_tiles(
+    const BatchGrid* __restrict__ grids, const int* __restrict__ tile_grid,
+    const int* __restrict__ tile_first, const int* __restrict__ counts) {
+  __shared__ double red[3 * WPB];
+  __shared__ int gid[WPB];
+  int b = blockIdx.x;
+  const int nblk = gridDim.x;
+  if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
+  const int t0 = b * WPB;
+  int cur = -1;
+  TileRed acc{kDtReset, 0.0, 0.0};
+  wave_tiles_step<R>(grids, tile_grid, tile_first, t0, min(t0 + WPB, counts[0]), red, gid, acc, cur);
+  if (threadIdx.x == 0 && cur >= 0) batch_publish(grids[cur], acc);
 }
 
 // Closes one batch step for each grid: t += dt (cuf_kernel.cuf:792-794),
@@ -876,8 +1131,12 @@ __global__ void k_batch_book(const BatchGrid* __restrict__ grids, const int* __r
 template <class R, int TX, int TY, int DIM, int MODE>
 static hipError_t launch_one(hipStream_t s, const FusedArgs& a) {
   if constexpr (DIM == 2) {  // 2-D grids march rows whatever the tile shape
-    hipLaunchKernelGGL((k_rows<R, MODE>), dim3((unsigned)tiles_of(a.g, 0, 0, a.kchunk)), dim3(RW), 0, s,
-                       a);
+    if (alone_wave_rows(a.g)) {
+      const int nb = (wave_row_tiles(a.g, a.kchunk) + WPB - 1) / WPB;
+      hipLaunchKernelGGL((k_wrows<R, MODE>), dim3((unsigned)nb), dim3(RW), 0, s, a);
+    } else {
+      hipLaunchKernelGGL((k_rows<R, MODE>), dim3((unsigned)row_tiles(a.g, a.kchunk)), dim3(RW), 0, s, a);
+    }
   } else {
     const int ntx = (a.g.nx + TX - 1) / TX, nty = (a.g.ny + TY - 1) / TY;
     const int nkz = (a.g.nz + a.kchunk - 1) / a.kchunk;
@@ -917,19 +1176,28 @@ static hipError_t launch_prec(hipStream_t s, const FusedArgs& a, int mode, int t
 
 template <class R, int TX, int TY>
 static hipError_t launch_batch_tile(hipStream_t s, const BatchLaunch& L) {
-  constexpr bool rows = TX == 32 && TY == 8;  // 2-D blocks: RW-wide row segments
-  if (L.per_tile) {
-    if (L.dim == 3)
-      hipLaunchKernelGGL((k_fused_tiles<R, TX, TY, 3>), dim3(L.nblocks), dim3(TX * TY), 0, s, L.grids,
-                         L.tile_grid, L.tile_first);
-    else if constexpr (rows)
-      hipLaunchKernelGGL((k_fused_tiles<R, TX, TY, 2>), dim3(L.nblocks), dim3(TX * TY), 0, s, L.grids,
-                         L.tile_grid, L.tile_first);
-  } else if (L.dim == 3) {
+  if (L.dim == 2) {  // wave rows (FP64) or block rows (mixed), whatever the tile shape
+    static_assert(batch_wave_rows(0) && !batch_wave_rows(1), "FP64 = double, mixed = float");
+    if constexpr (sizeof(R) == 8) {
+      if (L.per_tile)
+        hipLaunchKernelGGL((k_wrow_tiles<R>), dim3(L.nblocks), dim3(RW), 0, s, L.grids, L.tile_grid,
+                           L.tile_first, L.counts);
+      else
+        hipLaunchKernelGGL((k_wrow_batch<R>), dim3(L.nblocks), dim3(RW), 0, s, L.grids, L.tile_grid,
+                           L.tile_first, L.blk_first);
+    } else {
+      if (L.per_tile)
+        hipLaunchKernelGGL((k_brow_tiles<R>), dim3(L.nblocks), dim3(RW), 0, s, L.grids, L.tile_grid,
+                           L.tile_first);
+      else
+        hipLaunchKernelGGL((k_brow_batch<R>), dim3(L.nblocks), dim3(RW), 0, s, L.grids, L.tile_grid,
+                           L.tile_first, L.blk_first);
+    }
+  } else if (L.per_tile) {
+    hipLaunchKernelGGL((k_fused_tiles<R, TX, TY, 3>), dim3(L.nblocks), dim3(TX * TY), 0, s, L.grids,
+                       L.tile_grid, L.tile_first);
+  } else {
     hipLaunchKernelGGL((k_fused_batch<R, TX, TY, 3>), dim3(L.nblocks), dim3(TX * TY), 0, s, L.grids,
-                       L.tile_grid, L.tile_first, L.blk_first);
-  } else if constexpr (rows) {
-    hipLaunchKernelGGL((k_fused_batch<R, TX, TY, 2>), dim3(L.nblocks), dim3(TX * TY), 0, s, L.grids,
                        L.tile_grid, L.tile_first, L.blk_first);
   }
   hipError_t e = hipGetLastError();
@@ -940,7 +1208,7 @@ static hipError_t launch_batch_tile(hipStream_t s, const BatchLaunch& L) {
 
 hipError_t launch_batch(hipStream_t s, const BatchLaunch& L, int precision, int tile_x, int tile_y) {
   if (L.nblocks <= 0 || L.cap_grids <= 0) return hipSuccess;
-  if (L.dim == 2) {  // row march: the tile shape does not apply
+  if (L.dim == 2) {  // wave rows: the tile shape does not apply
     tile_x = 32;
     tile_y = 8;
   }

@@ -57,14 +57,15 @@ def test_cfg1_2d_parity(nsteps, bound, order):
     assert np.all(e <= bound), e
 
 
-@pytest.mark.parametrize("kchunk", [1, 2, 5, 64])
-def test_row_chunks_agree_2d(kchunk):
-    """2-D grids march 254-column row segments up chunks of kchunk rows (600
-    columns: segments of 254, 254 and 92).  The row below a chunk and the
-    face above it take the owned rows' code, so any chunking gives the same
-    bits."""
-    case(f"structured 2-D 600x70 kchunk={kchunk} vs 16, steps=20")
-    g = mesh.grid_2d(600, 70)
+@pytest.mark.parametrize("nx,kchunk", [(600, 1), (600, 2), (600, 5), (600, 64), (1100, 1), (1100, 7)])
+def test_row_chunks_agree_2d(nx, kchunk):
+    """2-D grids march row segments up chunks of kchunk rows: 62-column wave
+    rows up to 1024 columns (600: nine full segments and one of 42), 254-column
+    block rows beyond (1100: four full and one of 84).  The row below a chunk
+    and the face above it take the owned rows' code, so any chunking gives
+    the same bits."""
+    case(f"structured 2-D {nx}x70 kchunk={kchunk} vs 16, steps=20")
+    g = mesh.grid_2d(nx, 70)
     W = mesh.heliosphere_state(g)
     ref, tr, _ = gpu_run(g, W, 20, kchunk=16)
     cp, t, _ = gpu_run(g, W, 20, kchunk=kchunk)
@@ -188,10 +189,12 @@ def test_structured_equals_dropin_on_the_same_grid(shape, nsteps, order):
         assert np.all(e[:5] <= 1e-10), e
 
 
-# 2-D row segments own 254 columns: 254, 255 and 509 columns end a segment
-# exactly, leave one column to a second segment, and to a third
-TINY = [(1, 1, 1), (3, 1, 1), (1, 5, 1), (2, 2, 2), (1, 1, 7), (2, 3, 4), (33, 9, 1), (254, 3, 1),
-        (255, 2, 1), (509, 2, 1)]
+# 2-D wave rows own 62 columns per segment: 62, 63 and 125 columns end a
+# segment exactly, leave one column to a second segment, and to a third;
+# 254, 255 and 509 the same in 3-4 segments; 1100 columns march 254-column
+# block rows (four full segments and one of 84)
+TINY = [(1, 1, 1), (3, 1, 1), (1, 5, 1), (2, 2, 2), (1, 1, 7), (2, 3, 4), (33, 9, 1), (62, 3, 1),
+        (63, 2, 1), (125, 2, 1), (254, 3, 1), (255, 2, 1), (509, 2, 1), (1100, 3, 1)]
 
 
 def vec_err(a, b):
