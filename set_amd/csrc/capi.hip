@@ -301,9 +301,11 @@ UCfg ucfg(const setmhd_config& k) {
 }
 
 // planes (3-D) or rows (2-D) one tile marches, the configured value if set.
-// 3-D: enough blocks for about four per CU with 32x8 columns, within [8, 32]
-// planes (each chunk re-reads two boundary planes).  2-D in a batch: 16
-// rows (the grids fill the chip together).  A 2-D grid alone: a tile
+// Results do not depend on it (the marches run a chunk's boundary rows and
+// planes through their loop bodies).  3-D alone: enough blocks for about
+// four per CU with 32x8 columns, within [8, 32] planes (each chunk re-reads
+// two boundary planes).  In a batch (the grids fill the chip together):
+// 24 planes in 3-D, 16 rows in 2-D.  A 2-D grid alone: a tile
 // re-reads one row below its chunk and starts up in about two row-times,
 // and the launch runs in rounds of the resident tiles (two blocks per CU,
 // the FP64 occupancy; a third mixed-form block adds little; a block is one
@@ -312,6 +314,7 @@ UCfg ucfg(const setmhd_config& k) {
 // among those whose tiles fill half a round (one row when none does).
 int grid_kchunk(const setmhd_ctx* c, int nx, int ny, int nz, bool batch) {
   if (c->cfg.kchunk > 0) return c->cfg.kchunk;
+  if (batch && nz > 1) return 24;
   if (nz <= 1) {
     if (batch) return 16;
     const bool waves = nx <= kWaveRowsMaxNx;

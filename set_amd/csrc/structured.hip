@@ -267,7 +267,7 @@ struct TileRed {
   double dt, rd, rr;
 };
 
-// One tile (TX x TY cells, all planes of one z-chunk in 3-D) of one grid:
+// One 3-D tile (TX x TY columns, the planes of one z-chunk) of one grid:
 // stage the planes, form the faces, update the cells, and fold the tile's
 // CFL candidate into the grid's next dt slot.  book: this tile also does the
 // step's bookkeeping (t += dt, re-arm the dt slot after next).
@@ -275,6 +275,7 @@ template <class R, int TX, int TY, int DIM, int MODE, bool DMA = false>
 __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool book,
                                               Smem<R, TX, TY>& sm, double* red,
                                               Stage<TX, TY>* st = nullptr) {
+  static_assert(DIM == 3, "2-D grids march rows (row_tile, wrow_march)");
   constexpr int NH = 2 * TX + 2 * TY;
   constexpr int NW = TX * TY / 64;  // waves per block
   static_assert(TX <= 64 && TY <= 64 && NW >= 2 && NH <= (NW - 1) * 64 + 64, "extra-work layout");
@@ -296,8 +297,8 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
   const int i = i0 + tx, j = j0 + ty;
   const bool own = i < g.nx && j < g.ny;
   const int ic = min(i, g.nx - 1), jc = min(j, g.ny - 1);
-  const int k0 = DIM == 3 ? kc * a.kchunk : 0;
-  const int k1 = DIM == 3 ? min(k0 + a.kchunk, g.nz) : 1;
+  const int k0 = kc * a.kchunk;
+  const int k1 = min(k0 + a.kchunk, g.nz);
   const unsigned plane = (unsigned)g.nx * (unsigned)g.ny;
   const unsigned col0 = (unsigned)ic + (unsigned)g.nx * (unsigned)jc;
   const double dt = a.dt.slot[a.cur];
@@ -326,21 +327,11 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
   Side<R> zp;
   bool ok = true;
 
-  if (DIM == 3) {  // z-face data of the plane below the chunk
-    const int kb = max(k0 - 1, 0);
-    double U[8];
-    load8(a.Uf, g.nc, col0 + plane * kb, U);
-    CellQ<R> q;
-    derive<R, true>(U, q);
-    zp = side_of(q, 2);
-  }
-
   // Software pipeline (FP64): the next plane's own and halo cells are loaded
   // while the current plane's faces are formed.  The mixed form skips it:
   // its 32 extra VGPRs would cost a wave per SIMD.
   // With DMA (3-D, even nx) the next plane goes global -> LDS directly
   // while the faces are formed, using no registers.
-  static_assert(!DMA || DIM == 3, "plane DMA is for the z-march");
   constexpr bool PF = sizeof(R) == 8 && !DMA;
   const unsigned hcol = (unsigned)hgi + (unsigned)g.nx * (unsigned)hgj;
   // staged positions of this lane's own and halo cells (clamped as above)
@@ -355,13 +346,17 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
   // Ukeep: U^n of this column's previous plane, kept in registers (a
   // re-read would make its wait drain the next plane's prefetch as well)
   double Unx[8], Hnx[8], Ukeep[8] = {};
-  if constexpr (PF) {
-    load8(a.Uf, g.nc, col0 + plane * k0, Unx);
-    if (has_h) load8(a.Uf, g.nc, hcol + plane * k0, Hnx);
-  }
+  if constexpr (PF) load8(a.Uf, g.nc, col0 + plane * (unsigned)max(k0 - 1, 0), Unx);
 
-  for (int k = k0; k < k1; ++k) {
-    const unsigned idx = col0 + plane * k;
+  // Planes k0 - 1 (the plane below the chunk: its z-side only) through k1
+  // (the closing z-face above the chunk's last plane, from plane
+  // min(k1, nz - 1)) take the same code as the owned planes, so the result
+  // does not depend on the chunking (the loop is not peeled: a peeled copy
+  // of the flux code may contract multiply-adds differently).
+#pragma clang loop unroll(disable)
+  for (int k = k0 - 1; k <= k1; ++k) {
+    const bool below = k < k0, top = k == k1;
+    const unsigned kk = (unsigned)min(max(k, 0), g.nz - 1);  // the plane read
     double Uc[8], Uh[8];
     if constexpr (PF) {
 #pragma unroll
@@ -369,18 +364,22 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
         Uc[v] = Unx[v];
         Uh[v] = Hnx[v];
       }
-      if (DIM == 3) {  // plane k+1 (the epilogue's plane when k+1 == k1)
+      if (!top) {  // plane k+1: own cells, and the halo unless it is the closing plane
         const int kn = min(k + 1, g.nz - 1);
         load8(a.Uf, g.nc, col0 + plane * kn, Unx);
         if (has_h && k + 1 < k1) load8(a.Uf, g.nc, hcol + plane * kn, Hnx);
       }
     } else if constexpr (DMA) {
-      if (has_h) load8(a.Uf, g.nc, hcol + plane * k, Uh);
+      if (below) {
+        load8(a.Uf, g.nc, col0 + plane * kk, Uc);
+      } else {
+        if (has_h && !top) load8(a.Uf, g.nc, hcol + plane * kk, Uh);
 #pragma unroll
-      for (int v = 0; v < 8; v++) Uc[v] = st->own[v][soy][sox];
+        for (int v = 0; v < 8; v++) Uc[v] = st->own[v][soy][sox];
+      }
     } else {
-      load8(a.Uf, g.nc, idx, Uc);
-      if (has_h) load8(a.Uf, g.nc, hcol + plane * k, Uh);
+      load8(a.Uf, g.nc, col0 + plane * kk, Uc);
+      if (has_h && !below && !top) load8(a.Uf, g.nc, hcol + plane * kk, Uh);
     }
     double Uprev[8];  // U^n of the cell below, updated once the z-face between them is known
 #pragma unroll
@@ -389,62 +388,46 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
       Ukeep[v] = Uc[v];
     }
     CellQ<R> qc;
-    derive<R, DIM == 3>(Uc, qc);
-    put_q(sm, ty + 1, tx + 1, qc);
-    if (has_h) {
-      CellQ<R> qh;
-      derive<R, false>(Uh, qh);
-      put_q(sm, hly, hlx, qh);
-    }
-    __syncthreads();
-    // every wave has read the staged plane: stage plane k+1 (the epilogue's
-    // plane when k+1 == k1)
-    if constexpr (DMA) dma_plane<TX, TY>(*st, a, min(k + 1, g.nz - 1), wid, dma_off);
+    derive<R, true>(Uc, qc);
+    const Side<R> zc = side_of(qc, 2);
+    if (!below) {
+      if (!top) {
+        put_q(sm, ty + 1, tx + 1, qc);
+        if (has_h) {
+          CellQ<R> qh;
+          derive<R, false>(Uh, qh);
+          put_q(sm, hly, hlx, qh);
+        }
+        __syncthreads();
+        // every wave has read the staged plane: stage plane k+1 (the closing
+        // plane when k+1 == k1)
+        if constexpr (DMA) dma_plane<TX, TY>(*st, a, min(k + 1, g.nz - 1), wid, dma_off);
 
-    // x- and y-faces of the plane.  The TY faces on the tile's far x edge
-    // go to lanes 0..TY-1 of the role NW-1 wave, the TX faces on the far y
-    // edge to lanes 0..TX-1 of the role NW-2 wave, each as a second
-    // iteration of its loop.
-    {
-      const int nx_it = (role == NW - 1 && lane < TY) ? 2 : 1;
-      for (int r = 0; r < nx_it; r++) {
-        const int row = r == 0 ? ty + 1 : lane + 1;
-        const int col = r == 0 ? tx : TX;
-        lds_face<0>(sm, row, col, row, col + 1, smax[0], ok);
+        // x- and y-faces of the plane.  The TY faces on the tile's far x edge
+        // go to lanes 0..TY-1 of the role NW-1 wave, the TX faces on the far y
+        // edge to lanes 0..TX-1 of the role NW-2 wave, each as a second
+        // iteration of its loop.
+        {
+          const int nx_it = (role == NW - 1 && lane < TY) ? 2 : 1;
+          for (int r = 0; r < nx_it; r++) {
+            const int row = r == 0 ? ty + 1 : lane + 1;
+            const int col = r == 0 ? tx : TX;
+            lds_face<0>(sm, row, col, row, col + 1, smax[0], ok);
+          }
+          const int ny_it = (role == NW - 2 && lane < TX) ? 2 : 1;
+          for (int r = 0; r < ny_it; r++) {
+            const int row = r == 0 ? ty : TY;
+            const int col = r == 0 ? tx + 1 : lane + 1;
+            lds_face<1>(sm, row, col, row + 1, col, smax[1], ok);
+          }
+        }
       }
-      const int ny_it = (role == NW - 2 && lane < TX) ? 2 : 1;
-      for (int r = 0; r < ny_it; r++) {
-        const int row = r == 0 ? ty : TY;
-        const int col = r == 0 ? tx + 1 : lane + 1;
-        lds_face<1>(sm, row, col, row + 1, col, smax[1], ok);
+      R Fz[8];
+      {  // z-face below this cell
+        R s;
+        ok &= hlld(zp, zc, Fz, s);
+        smax[2] = fmax(smax[2], (double)s);
       }
-    }
-
-    R Fz[8];
-    Side<R> zc;
-    if (DIM == 3) {  // z-face below this cell
-      zc = side_of(qc, 2);
-      R s;
-      ok &= hlld(zp, zc, Fz, s);
-      smax[2] = fmax(smax[2], (double)s);
-    }
-    // DMA: this wave's share of plane k+1 has landed.  Otherwise the
-    // prefetch, waited for here rather than at the loop's back edge, where
-    // the wait would also drain the finalize stores issued after it.
-    wait_vm();
-    __syncthreads();
-
-    // signed face sum in the face-slot order (x-, x+, y-, y+, z-, z+)
-    double acc[8];
-#pragma unroll
-    for (int v = 0; v < 8; v++) {
-      double s = -((double)sm.fx[v][ty][tx] * g.ax);
-      s = s + (double)sm.fx[v][ty][tx + 1] * g.ax;
-      s = s - (double)sm.fy[v][ty][tx] * g.ay;
-      s = s + (double)sm.fy[v][ty + 1][tx] * g.ay;
-      acc[v] = s;
-    }
-    if (DIM == 3) {
       // z-face: rotate frame (z, x, y) back to x, y, z
       double fz[8];
       fz[0] = Fz[0];
@@ -455,51 +438,31 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
       fz[7] = Fz[5];
       fz[5] = Fz[6];
       fz[6] = Fz[7];
-#pragma unroll
-      for (int v = 0; v < 8; v++) acc[v] = acc[v] - fz[v] * g.az;
-      if (k > k0 && own) {
+      // DMA: this wave's share of plane k+1 has landed.  Otherwise the
+      // prefetch, waited for here, before the finalize stores: a wait after
+      // them (or at the loop's back edge) would drain those stores as well.
+      wait_vm();
+      if (k > k0 && own) {  // update the cell below: its faces are all known
 #pragma unroll
         for (int v = 0; v < 8; v++) accp[v] = accp[v] + fz[v] * g.az;
         const double zcen = __dadd_rn(g.z0, __dmul_rn((double)(k - 1) + 0.5, g.dz));
-        finalize_with<MODE>(a, dt, idx - plane, held_cell(g, xc, yc, zcen), accp, Uprev, rs_d, rs_r);
+        finalize_with<MODE>(a, dt, col0 + plane * (unsigned)(k - 1), held_cell(g, xc, yc, zcen), accp, Uprev,
+                            rs_d, rs_r);
       }
+      if (!top) {
+        __syncthreads();
+        // signed face sum in the face-slot order (x-, x+, y-, y+, z-, z+)
 #pragma unroll
-      for (int v = 0; v < 8; v++) accp[v] = acc[v];
-      zp = zc;
-    } else if (own) {
-      finalize<MODE>(a, dt, idx, held_cell(g, xc, yc, 0.0), acc, rs_d, rs_r);
+        for (int v = 0; v < 8; v++) {
+          double s = -((double)sm.fx[v][ty][tx] * g.ax);
+          s = s + (double)sm.fx[v][ty][tx + 1] * g.ax;
+          s = s - (double)sm.fy[v][ty][tx] * g.ay;
+          s = s + (double)sm.fy[v][ty + 1][tx] * g.ay;
+          accp[v] = s - fz[v] * g.az;
+        }
+      }
     }
-  }
-
-  if (DIM == 3) {  // z-face above the chunk's last plane
-    if constexpr (DMA) {
-#pragma unroll
-      for (int v = 0; v < 8; v++) Unx[v] = st->own[v][soy][sox];
-    } else if constexpr (!PF) {
-      load8(a.Uf, g.nc, col0 + plane * min(k1, g.nz - 1), Unx);
-    }
-    CellQ<R> q;
-    derive<R, true>(Unx, q);  // plane min(k1, nz-1)
-    const Side<R> zc = side_of(q, 2);
-    R Fz[8], s;
-    ok &= hlld(zp, zc, Fz, s);
-    smax[2] = fmax(smax[2], (double)s);
-    if (own) {
-      double fz[8];
-      fz[0] = Fz[0];
-      fz[3] = Fz[1];
-      fz[1] = Fz[2];
-      fz[2] = Fz[3];
-      fz[4] = Fz[4];
-      fz[7] = Fz[5];
-      fz[5] = Fz[6];
-      fz[6] = Fz[7];
-#pragma unroll
-      for (int v = 0; v < 8; v++) accp[v] = accp[v] + fz[v] * g.az;
-      const double zcen = __dadd_rn(g.z0, __dmul_rn((double)(k1 - 1) + 0.5, g.dz));
-      finalize<MODE>(a, dt, col0 + plane * (k1 - 1),
-                     held_cell(g, xc, yc, zcen), accp, rs_d, rs_r);
-    }
+    zp = zc;
   }
   if (!ok) atomicOr(a.err, kErrRegion);
 
@@ -510,7 +473,7 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
   if (MODE != kModeHeun2) {
     cand = g.cfl * g.distx / smax[0];
     cand = fmin(cand, g.cfl * g.disty / smax[1]);
-    if (DIM == 3) cand = fmin(cand, g.cfl * g.distz / smax[2]);
+    cand = fmin(cand, g.cfl * g.distz / smax[2]);
     cand = fmin(cand, g.dt_cap);
     cand = wave_min(cand);
   }
