@@ -77,9 +77,11 @@ typedef struct setmhd_config {
   double r_hold_cell;  /* cells with |centre| < r held; ref 0.05 (:611)       */
   int32_t precision;   /* SETMHD_PREC_* (structured path)                    */
   int32_t device;      /* HIP device ordinal                                 */
-  int32_t tile_x;      /* structured tile width in cells (0 = default)       */
-  int32_t tile_y;      /* structured tile height (0 = default)               */
-  int32_t kchunk;      /* structured 3-D: z-planes per block (0 = default 32) */
+  int32_t tile_x;      /* structured 3-D tile tile_x x tile_y: 32x8, 16x16,  */
+  int32_t tile_y;      /* 64x4 or 16x8 (0 = 16x16 in FP64, 64x4 in the mixed
+                          form; 2-D grids march rows whatever the tile)       */
+  int32_t kchunk;      /* structured: z-planes (3-D) or rows (2-D) per tile
+                          (0 = chosen per grid; results do not depend on it) */
 } setmhd_config;
 
 typedef struct setmhd_ctx setmhd_ctx;

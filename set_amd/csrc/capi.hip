@@ -338,8 +338,14 @@ int grid_kchunk(const setmhd_ctx* c, int nx, int ny, int nz, bool batch) {
   return (int)std::min(32LL, std::max(8LL, kc));
 }
 
-int tile_x(const setmhd_ctx* c) { return c->cfg.tile_x > 0 ? c->cfg.tile_x : 32; }
-int tile_y(const setmhd_ctx* c) { return c->cfg.tile_y > 0 ? c->cfg.tile_y : 8; }
+// Default z-march tiles of a grid alone: 16x16 in FP64, 64x4 in the mixed
+// form (256^3: +1.5 % and +2 % against 32x8, the mixed form's plane-DMA shape)
+int tile_x(const setmhd_ctx* c) {
+  return c->cfg.tile_x > 0 ? c->cfg.tile_x : (c->cfg.precision == SETMHD_PREC_FP64 ? 16 : 64);
+}
+int tile_y(const setmhd_ctx* c) {
+  return c->cfg.tile_y > 0 ? c->cfg.tile_y : (c->cfg.precision == SETMHD_PREC_FP64 ? 16 : 4);
+}
 
 // drop-in kernels: the exact build, or the fast one under SETMHD_F_FAST
 template <class... A>
