@@ -234,6 +234,14 @@ int setmhd_batch_set_launch(setmhd_ctx *ctx, int32_t dim, int32_t tile_x, int32_
 /* Free and total device memory (admission control). */
 int setmhd_mem_info(setmhd_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes);
 
+/* Cell_par(1:8) (rho, u, v, w, p, Bx, By, Bz) of n cells, 1-based cell
+ * numbers, from the current state of a face-list mesh or a structured grid:
+ * one gather and one copy of 8 n doubles into out (cell after cell), the
+ * same values a download gives; *t the simulated time (may be NULL).  In
+ * place of the probe series' single-element device reads and their syncs
+ * (cuf_kernel.cuf:886-906). */
+int setmhd_probe(setmhd_ctx *ctx, int32_t n, const int32_t *cells, double *out, double *t);
+
 /* Message of the last failure of this context ("" if none). */
 int setmhd_last_error(const setmhd_ctx *ctx, char *buf, size_t n);
 

@@ -91,6 +91,8 @@ struct setmhd_ctx {
   // blocks per CU (0: one block per tile)
   int bt_x[2] = {32, 32}, bt_y[2] = {8, 8}, bt_bpc[2] = {0, 0};
   hipEvent_t ev_mid = nullptr;  // autotune: splits a probe between the two launches
+  void* d_probe = nullptr;      // setmhd_probe: cell ids, then 8 doubles per cell
+  int probe_cap = 0;
   int n_cu = 256;
 };
 
@@ -214,17 +216,17 @@ __global__ void k_prim_to_cons(const double* __restrict__ cp, long long nc, doub
   U[7 * nc + c] = bz;
 }
 
-__global__ void k_cons_to_prim(const double* __restrict__ U, long long nc, double* __restrict__ cp) {
+// Cell_par(1:8) primitives of cell c from the structured conserved state
+// (SoA, b in Heaviside-Lorentz units) into o[0..7]
+__device__ __forceinline__ void cons_to_prim_cell(const double* __restrict__ U, long long nc, long long c,
+                                                  double* __restrict__ o) {
 #pragma clang fp contract(off)
-  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
   const double spi4 = sqrt(kCpi4);
   double rho = U[c];
   double u = U[nc + c] / rho, v = U[2 * nc + c] / rho, w = U[3 * nc + c] / rho;
   double bx = U[5 * nc + c], by = U[6 * nc + c], bz = U[7 * nc + c];
   double p = (kGamma - 1.0) * (U[4 * nc + c] - 0.5 * rho * ((u * u + v * v) + w * w) -
                                0.5 * ((bx * bx + by * by) + bz * bz));
-  double* o = cp + 16 * c;
   o[0] = rho;
   o[1] = u;
   o[2] = v;
@@ -233,6 +235,27 @@ __global__ void k_cons_to_prim(const double* __restrict__ U, long long nc, doubl
   o[5] = bx * spi4;
   o[6] = by * spi4;
   o[7] = bz * spi4;
+}
+
+__global__ void k_cons_to_prim(const double* __restrict__ U, long long nc, double* __restrict__ cp) {
+  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  cons_to_prim_cell(U, nc, c, cp + 16 * c);
+}
+
+// Probes: Cell_par(1:8) of n cells (0-based ids) from the face-list state W
+// (8 primitives per cell) or, when W is null, the structured state U.
+__global__ void k_probe(const double* __restrict__ W, const double* __restrict__ U, long long nc,
+                        const int* __restrict__ cells, int n, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long c = cells[i];
+  if (W) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) out[8 * i + k] = W[8 * c + k];
+  } else {
+    cons_to_prim_cell(U, nc, c, out + 8 * i);
+  }
 }
 
 // AoS16 <-> AoS8 column split for the unstructured state
@@ -523,6 +546,7 @@ void setmhd_destroy(setmhd_ctx* c) {
   free_state(c);
   if (c->d_scal) hipFree(c->d_scal);
   if (c->d_err) hipFree(c->d_err);
+  if (c->d_probe) hipFree(c->d_probe);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->ev_side) hipEventDestroy(c->ev_side);
@@ -790,6 +814,39 @@ int setmhd_download(setmhd_ctx* c, double* cell_par, double* t) {
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   hipFree(stage);
   if (e != hipSuccess) return fail(c, SETMHD_ERR_HIP, std::string("download: ") + hipGetErrorString(e));
+  return setmhd_sync(c, t, nullptr);
+}
+
+int setmhd_probe(setmhd_ctx* c, int32_t n, const int32_t* cells, double* out, double* t) {
+  if (!c) return SETMHD_ERR_ARG;
+  if (c->kind != 1 && c->kind != 2) return fail(c, SETMHD_ERR_ARG, "probe: no mesh or grid uploaded");
+  if (n < 1 || !cells || !out) return fail(c, SETMHD_ERR_ARG, "probe: no cells or null output");
+  std::vector<int> ids(n);
+  for (int32_t i = 0; i < n; i++) {
+    if (cells[i] < 1 || cells[i] > c->n_cell) return fail(c, SETMHD_ERR_ARG, "probe: cell number out of range");
+    ids[i] = cells[i] - 1;
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  if (n > c->probe_cap) {  // grow-only: ids, then 8 doubles per cell
+    if (c->d_probe) (void)hipFree(c->d_probe);
+    c->d_probe = nullptr;
+    c->probe_cap = 0;
+    if (hipMalloc(&c->d_probe, (size_t)n * (sizeof(int) + 8 * sizeof(double)) + 8) != hipSuccess)
+      return fail(c, SETMHD_ERR_NOMEM, "probe: hipMalloc");
+    c->probe_cap = n;
+  }
+  int* d_ids = (int*)c->d_probe;
+  double* d_out = (double*)((char*)c->d_probe + (size_t)c->probe_cap * sizeof(int));
+  d_out = (double*)(((uintptr_t)d_out + 7) & ~(uintptr_t)7);
+  HIPCHK(c, hipMemcpyAsync(d_ids, ids.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  const double* W = c->kind == 1 ? c->d_W : nullptr;
+  const double* U = c->kind == 2 ? ((c->cfg.time_scheme == SETMHD_TIME_EULER && c->cur_buf == 1) ? c->d_U2 : c->d_U)
+                                 : nullptr;
+  hipLaunchKernelGGL(k_probe, dim3((n + 255) / 256), dim3(256), 0, c->stream, W, U, (long long)c->n_cell, d_ids, n,
+                     d_out);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(out, d_out, (size_t)n * 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return setmhd_sync(c, t, nullptr);
 }
 

@@ -9,7 +9,15 @@
 ! the output file (Save_Storage, :163-189).  Every kernel launch goes
 ! through the C ABI (module setmhd).
 !
-! usage: fcmhd_driver input.bin output.bin all_step [print_step] [restart.bin] [cartesian]
+! Observability, in the text formats of FCMHD.f90:6-119: with a probe file
+! (one 1-based cell number per line) every print_step the probed densities
+! are appended to 1D_rhoT.txt (write_T_rho) through setmhd_probe, one
+! gather instead of a read per cell; with 'dump' every print_step writes
+! output_data_<k>.txt (x, y, rho: Print_mult) and the end output_data.txt
+! (x, y, rho, u, v, p, |V|: Print).
+!
+! usage: fcmhd_driver input.bin output.bin all_step [print_step] [restart.bin|-]
+!                     [cartesian|reference] [probes.txt|-] [dump]
 program fcmhd_driver
   use iso_c_binding
   use setmhd
@@ -23,8 +31,11 @@ program fcmhd_driver
   integer(4), allocatable :: host_Gran_neighbour(:, :), host_Gran_neighbour_TVD(:, :)
   integer(4), allocatable :: host_Gran_type(:)
   real(8), allocatable :: host_Gran_POTOK(:, :)
-  character(len=512) :: fin, fout, arg, frestart
-  integer :: all_step, print_step, step, chunk, nargs, u
+  character(len=512) :: fin, fout, arg, frestart, fprobes
+  integer :: all_step, print_step, step, chunk, nargs, u, ios, nprobe, i, k_dump, itmp
+  integer(4), allocatable :: probes(:)
+  real(8), allocatable :: pv(:, :)
+  logical :: dump
   logical :: cartesian
   type(setmhd_config) :: cfg
   type(c_ptr) :: ctx
@@ -32,7 +43,8 @@ program fcmhd_driver
 
   nargs = command_argument_count()
   if (nargs < 3) then
-    print *, 'usage: fcmhd_driver input.bin output.bin all_step [print_step] [restart.bin] [cartesian]'
+    print *, 'usage: fcmhd_driver input.bin output.bin all_step [print_step] [restart.bin|-] ', &
+             '[cartesian|reference] [probes.txt|-] [dump]'
     stop 2
   end if
   call get_command_argument(1, fin)
@@ -46,11 +58,36 @@ program fcmhd_driver
   end if
   frestart = ''
   if (nargs >= 5) call get_command_argument(5, frestart)
+  if (trim(frestart) == '-') frestart = ''
   cartesian = .false.
   if (nargs >= 6) then
     call get_command_argument(6, arg)
     cartesian = trim(arg) == 'cartesian'
   end if
+  fprobes = ''
+  if (nargs >= 7) call get_command_argument(7, fprobes)
+  if (trim(fprobes) == '-') fprobes = ''
+  dump = .false.
+  if (nargs >= 8) then
+    call get_command_argument(8, arg)
+    dump = trim(arg) == 'dump'
+  end if
+  nprobe = 0
+  if (len_trim(fprobes) > 0) then
+    open(newunit=u, file=trim(fprobes), action='read', status='old')
+    do
+      read(u, *, iostat=ios) itmp
+      if (ios /= 0) exit
+      nprobe = nprobe + 1
+    end do
+    rewind(u)
+    allocate(probes(nprobe), pv(8, nprobe))
+    do i = 1, nprobe
+      read(u, *) probes(i)
+    end do
+    close(u)
+  end if
+  k_dump = 0
 
   ! ---- Set_Storage --------------------------------------------------------
   open(newunit=u, file=trim(fin), form='unformatted', access='stream', action='read', status='old')
@@ -118,6 +155,17 @@ program fcmhd_driver
     st = setmhd_sync(ctx, t, dt_next)
     call setmhd_check(ctx, st, 'sync')
     print *, 'Step = ', step, '  step_time = ', dt_next, ' all time = ', t
+    if (nprobe > 0) then
+      st = setmhd_probe(ctx, int(nprobe, c_int32_t), probes, pv, t)
+      call setmhd_check(ctx, st, 'probe')
+      call append_probes(t, pv(1, :))
+    end if
+    if (dump) then
+      st = setmhd_download(ctx, host_Cell_par, t)
+      call setmhd_check(ctx, st, 'download')
+      k_dump = k_dump + 1
+      call dump_density(k_dump, t)
+    end if
   end do
 
   ! ---- Send_data_to_Host + Save_Storage --------------------------------------
@@ -130,4 +178,57 @@ program fcmhd_driver
   write(u) 321.0_8
   close(u)
   print *, 'saved ', trim(fout), ' time = ', host_time_all
+  if (dump) call dump_state()
+
+contains
+
+  ! SET's dimensional units: the reference scales with default-real
+  ! (single-precision) literals, so these are their single-precision values.
+  real(8) function sx(v)
+    real(8), intent(in) :: v
+    sx = v * real(319.319, 8)
+  end function
+
+  ! probe series line, the layout of write_T_rho (FCMHD.f90:98-119)
+  subroutine append_probes(tt, rho)
+    real(8), intent(in) :: tt, rho(:)
+    integer :: w
+    open(newunit=w, file='1D_rhoT.txt', position='append', action='write', status='unknown')
+    write(w, '(18es23.15)') tt * real(146.262, 8), rho * real(0.073, 8)
+    close(w)
+  end subroutine
+
+  ! density dump, the layout of Print_mult (FCMHD.f90:52-96)
+  subroutine dump_density(k, tt)
+    integer, intent(in) :: k
+    real(8), intent(in) :: tt
+    character(len=64) :: name
+    integer :: w, c
+    write(name, '(A, I0, A)') 'output_data_', k, '.txt'
+    open(newunit=w, file=trim(name), status='replace', action='write')
+    write(w, '(A)') 'TITLE = "HP"'
+    write(w, '(A)') 'VARIABLES = x, y, rho'
+    write(w, '(A, ES12.5)') 'ZONE T="HP", STRANDID=1, SOLUTIONTIME=', tt * real(146.262, 8)
+    do c = 1, host_N_cell
+      write(w, '(3(ES15.7, 2X))') sx(host_Cell_center(1, c)), sx(host_Cell_center(2, c)), &
+          host_Cell_par(1, c) * real(0.073, 8)
+    end do
+    close(w)
+  end subroutine
+
+  ! final state, the layout of Print (FCMHD.f90:6-50)
+  subroutine dump_state()
+    integer :: w, c
+    real(8) :: uu, vv2
+    open(newunit=w, file='output_data.txt', status='replace', action='write')
+    write(w, '(A)') 'TITLE = "HP"  VARIABLES = x, y, rho, u, v, p, |V|'
+    do c = 1, host_N_cell
+      uu = host_Cell_par(2, c) * real(10.3565, 8)
+      vv2 = host_Cell_par(3, c) * real(10.3565, 8)
+      write(w, '(6(ES15.7, 2X))') sx(host_Cell_center(1, c)), sx(host_Cell_center(2, c)), &
+          host_Cell_par(1, c) * real(0.073, 8), uu, vv2, host_Cell_par(5, c) * real(1.30962, 8), &
+          sqrt(uu**2 + vv2**2)
+    end do
+    close(w)
+  end subroutine
 end program fcmhd_driver

@@ -82,6 +82,15 @@ module setmhd
       real(c_double), intent(out) :: t
     end function
 
+    integer(c_int) function setmhd_probe(ctx, n, cells, out, t) bind(C, name="setmhd_probe")
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: n
+      integer(c_int32_t), intent(in) :: cells(*)
+      real(c_double), intent(inout) :: out(*)
+      real(c_double), intent(out) :: t
+    end function
+
     integer(c_int) function setmhd_last_error(ctx, buf, n) bind(C, name="setmhd_last_error")
       import :: c_int, c_ptr, c_char, c_size_t
       type(c_ptr), value :: ctx

@@ -37,7 +37,7 @@ EXPORTS = (
     "setmhd_config_reference", "setmhd_config_cartesian", "setmhd_create", "setmhd_destroy",
     "setmhd_upload_mesh", "setmhd_upload_structured", "setmhd_set_dt", "setmhd_dt_from_state",
     "setmhd_step", "setmhd_step_timed", "setmhd_sync", "setmhd_download",
-    "setmhd_download_potok", "setmhd_sizes", "setmhd_residual", "setmhd_last_error",
+    "setmhd_download_potok", "setmhd_probe", "setmhd_sizes", "setmhd_residual", "setmhd_last_error",
     "setmhd_batch_add", "setmhd_batch_remove", "setmhd_batch_step", "setmhd_batch_step_async",
     "setmhd_batch_poll", "setmhd_batch_wait", "setmhd_batch_query",
     "setmhd_batch_download", "setmhd_batch_autotune", "setmhd_batch_set_launch", "setmhd_mem_info",
@@ -87,6 +87,7 @@ def load():
     L.setmhd_sync.argtypes = [P, PD, PD]
     L.setmhd_download.argtypes = [P, PD, PD]
     L.setmhd_download_potok.argtypes = [P, PD]
+    L.setmhd_probe.argtypes = [P, I32, PI, PD, PD]
     L.setmhd_sizes.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     L.setmhd_residual.argtypes = [P, PD]
     L.setmhd_last_error.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t]
@@ -239,6 +240,16 @@ class Context:
         p = np.zeros((self.n_gran, 9))
         self._check(self._L.setmhd_download_potok(self._h, _pd(p)))
         return p
+
+    def probe(self, cells):
+        """Cell_par(1:8) of the given 1-based cell numbers from the current
+        state (one gather and one small copy); returns (n x 8 array, t)."""
+        ids = np.ascontiguousarray(cells, np.int32).reshape(-1)
+        out = np.zeros((ids.size, 8))
+        t = ctypes.c_double()
+        self._check(self._L.setmhd_probe(self._h, int(ids.size), ids.ctypes.data_as(PI), _pd(out),
+                                         ctypes.byref(t)))
+        return out, t.value
 
     # -- batches of structured grids ---------------------------------------
     def batch_add(self, grid, cell_par, t0=0.0):

@@ -190,3 +190,36 @@ def test_fortran_driver(tmp_path, mode):
         assert np.all(err <= 1e-12), err
     np.testing.assert_array_equal(cp[:, 8:], W[:, 8:])
     assert t == pytest.approx(s.t, rel=1e-13)
+
+
+def test_fortran_driver_outputs(tmp_path):
+    """The driver's observability outputs: the probe series (setmhd_probe,
+    write_T_rho layout) and the Tecplot dumps (Print_mult and Print layouts)
+    hold the run's state; the dumps equal set_amd.tecplot's text."""
+    import os
+    import subprocess
+
+    from set_amd import fcmhd, tecplot
+    case("fortran driver probes and dumps")
+    exe = os.path.join(os.path.dirname(solver.HERE), "set_amd", "fortran", "fcmhd_driver")
+    fm, W = helio_mesh(60, 30)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    fcmhd.write_input(fin, fm, W)
+    cells = [1, 100, 1800]
+    (tmp_path / "probes.txt").write_text("".join(f"{c}\n" for c in cells))
+    r = subprocess.run([exe, str(fin), str(fout), "12", "5", "-", "reference", "probes.txt", "dump"],
+                       capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    t, cp = fcmhd.fill_data(fout, fm.n_cell)
+    series = (tmp_path / "1D_rhoT.txt").read_text().splitlines()
+    assert len(series) == 3  # after steps 5, 10 and 12
+    last = [float(v) for v in series[-1].split()]
+    np.testing.assert_allclose(last, [t * tecplot.SCALE_T] + list(cp[np.array(cells) - 1, 0] * tecplot.SCALE_RHO),
+                               rtol=1e-14)
+    tecplot.write_density(tmp_path / "py_density.txt", fm.cell_center, cp, t)
+    tecplot.write_state(tmp_path / "py_state.txt", fm.cell_center, cp)
+    for f_out, py in (("output_data_3.txt", "py_density.txt"), ("output_data.txt", "py_state.txt")):
+        a = [l.rstrip() for l in (tmp_path / f_out).read_text().splitlines()]
+        b = [l.rstrip() for l in (tmp_path / py).read_text().splitlines()]
+        assert a == b, (f_out, next((x, y) for x, y in zip(a, b) if x != y) if len(a) == len(b) else len(a))
