@@ -234,6 +234,16 @@ int setmhd_batch_set_launch(setmhd_ctx *ctx, int32_t dim, int32_t tile_x, int32_
 /* Free and total device memory (admission control). */
 int setmhd_mem_info(setmhd_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes);
 
+/* Snapshot of Cell_par and the time for a dump that overlaps the next steps
+ * (the periodic Send_data_to_Host of cuf_kernel.cuf:912-919 blocks instead):
+ * download_async queues the conversion behind the queued steps and the copy
+ * to pinned host memory on a second stream, and returns; steps queued after
+ * it do not change the snapshot.  download_wait waits for that copy and
+ * fills cell_par (16 doubles per cell) and *t (may be NULL).  One snapshot
+ * at a time: download_async fails while one is not yet collected. */
+int setmhd_download_async(setmhd_ctx *ctx);
+int setmhd_download_wait(setmhd_ctx *ctx, double *cell_par, double *t);
+
 /* Cell_par(1:8) (rho, u, v, w, p, Bx, By, Bz) of n cells, 1-based cell
  * numbers, from the current state of a face-list mesh or a structured grid:
  * one gather and one copy of 8 n doubles into out (cell after cell), the

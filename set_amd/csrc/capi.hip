@@ -93,6 +93,13 @@ struct setmhd_ctx {
   hipEvent_t ev_mid = nullptr;  // autotune: splits a probe between the two launches
   void* d_probe = nullptr;      // setmhd_probe: cell ids, then 8 doubles per cell
   int probe_cap = 0;
+  // setmhd_download_async: device and pinned snapshot (Cell_par, then t), grow-only
+  double* snap_dev = nullptr;
+  double* snap_host = nullptr;
+  size_t snap_cap = 0;  // doubles
+  long long snap_n = 0;  // cells of the snapshot in flight
+  bool snap_pending = false;
+  hipEvent_t ev_snapc = nullptr, ev_snap = nullptr;
   int n_cu = 256;
 };
 
@@ -173,6 +180,10 @@ void free_batch(setmhd_ctx* c) {
 }
 
 void free_state(setmhd_ctx* c) {
+  if (c->snap_pending) {  // a snapshot's copy may still read its staging
+    hipEventSynchronize(c->ev_snap);
+    c->snap_pending = false;
+  }
   for (double** p : {&c->d_W, &c->d_W2, &c->d_H, &c->d_potok, &c->d_U, &c->d_U2, &c->d_H16}) {
     if (*p) hipFree(*p);
     *p = nullptr;
@@ -480,6 +491,11 @@ int unstructured_step(setmhd_ctx* c) {
 
 }  // namespace
 
+namespace {
+int ensure_side(setmhd_ctx* c);  // batch_impl.inc: the context's second stream
+hipError_t convert_state(setmhd_ctx* c, double* dst);
+}  // namespace
+
 extern "C" {
 
 void setmhd_config_reference(setmhd_config* k) {
@@ -547,6 +563,11 @@ void setmhd_destroy(setmhd_ctx* c) {
   if (c->d_scal) hipFree(c->d_scal);
   if (c->d_err) hipFree(c->d_err);
   if (c->d_probe) hipFree(c->d_probe);
+  if (c->side) hipStreamSynchronize(c->side);
+  if (c->snap_dev) hipFree(c->snap_dev);
+  if (c->snap_host) hipHostFree(c->snap_host);
+  if (c->ev_snapc) hipEventDestroy(c->ev_snapc);
+  if (c->ev_snap) hipEventDestroy(c->ev_snap);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->ev_side) hipEventDestroy(c->ev_side);
@@ -802,19 +823,80 @@ int setmhd_download(setmhd_ctx* c, double* cell_par, double* t) {
   double* stage = nullptr;
   int st;
   if ((st = dalloc(c, &stage, (size_t)nc * 16))) return st;
-  if (c->kind == 1) {
-    hipLaunchKernelGGL(k_join16, dim3(nblocks(nc)), dim3(256), 0, c->stream, c->d_W, c->d_H, nc, stage);
-  } else {
-    const double* U = (c->cfg.time_scheme == SETMHD_TIME_EULER && c->cur_buf == 1) ? c->d_U2 : c->d_U;
-    hipLaunchKernelGGL(k_cons_to_prim, dim3(nblocks(nc)), dim3(256), 0, c->stream, U, nc, stage);
-    hipLaunchKernelGGL(k_put_h, dim3(nblocks(nc)), dim3(256), 0, c->stream, c->d_H16, nc, stage);
-  }
-  hipError_t e = hipGetLastError();
+  hipError_t e = convert_state(c, stage);
   if (e == hipSuccess) e = hipMemcpyAsync(cell_par, stage, (size_t)nc * 16 * 8, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   hipFree(stage);
   if (e != hipSuccess) return fail(c, SETMHD_ERR_HIP, std::string("download: ") + hipGetErrorString(e));
   return setmhd_sync(c, t, nullptr);
+}
+
+}  // extern "C"
+
+namespace {
+// Cell_par of the current state into dst (16 doubles per cell) on the
+// library's stream.
+hipError_t convert_state(setmhd_ctx* c, double* dst) {
+  const long long nc = c->n_cell;
+  if (c->kind == 1) {
+    hipLaunchKernelGGL(k_join16, dim3(nblocks(nc)), dim3(256), 0, c->stream, c->d_W, c->d_H, nc, dst);
+  } else {
+    const double* U = (c->cfg.time_scheme == SETMHD_TIME_EULER && c->cur_buf == 1) ? c->d_U2 : c->d_U;
+    hipLaunchKernelGGL(k_cons_to_prim, dim3(nblocks(nc)), dim3(256), 0, c->stream, U, nc, dst);
+    hipLaunchKernelGGL(k_put_h, dim3(nblocks(nc)), dim3(256), 0, c->stream, c->d_H16, nc, dst);
+  }
+  return hipGetLastError();
+}
+}  // namespace
+
+extern "C" {
+
+int setmhd_download_async(setmhd_ctx* c) {
+  if (!c) return SETMHD_ERR_ARG;
+  if (c->kind != 1 && c->kind != 2) return fail(c, SETMHD_ERR_ARG, "download_async: no mesh or grid uploaded");
+  if (c->snap_pending) return fail(c, SETMHD_ERR_ARG, "download_async: the previous snapshot was not collected");
+  HIPCHK(c, hipSetDevice(c->device));
+  int st = ensure_side(c);
+  if (st) return st;
+  const size_t nc = (size_t)c->n_cell, nd = nc * 16 + 1;
+  if (nd > c->snap_cap) {
+    HIPCHK(c, hipStreamSynchronize(c->side));
+    if (c->snap_dev) (void)hipFree(c->snap_dev);
+    if (c->snap_host) (void)hipHostFree(c->snap_host);
+    c->snap_dev = c->snap_host = nullptr;
+    c->snap_cap = 0;
+    if (hipMalloc((void**)&c->snap_dev, nd * 8) != hipSuccess)
+      return fail(c, SETMHD_ERR_NOMEM, "download_async: hipMalloc");
+    if (hipHostMalloc((void**)&c->snap_host, nd * 8, hipHostMallocDefault) != hipSuccess)
+      return fail(c, SETMHD_ERR_NOMEM, "download_async: hipHostMalloc");
+    c->snap_cap = nd;
+  }
+  if (!c->ev_snapc) HIPCHK(c, hipEventCreateWithFlags(&c->ev_snapc, hipEventDisableTiming));
+  if (!c->ev_snap) HIPCHK(c, hipEventCreateWithFlags(&c->ev_snap, hipEventDisableTiming));
+  // the conversion and the time follow the queued steps on the library's
+  // stream (later steps cannot overwrite the state before it is taken); the
+  // copy to the host runs on the side stream beside those later steps
+  HIPCHK(c, convert_state(c, c->snap_dev));
+  HIPCHK(c, hipMemcpyAsync(c->snap_dev + nc * 16, c->d_scal + 3, sizeof(double), hipMemcpyDeviceToDevice,
+                           c->stream));
+  HIPCHK(c, hipEventRecord(c->ev_snapc, c->stream));
+  HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_snapc, 0));
+  HIPCHK(c, hipMemcpyAsync(c->snap_host, c->snap_dev, nd * 8, hipMemcpyDeviceToHost, c->side));
+  HIPCHK(c, hipEventRecord(c->ev_snap, c->side));
+  c->snap_n = (long long)nc;
+  c->snap_pending = true;
+  return SETMHD_OK;
+}
+
+int setmhd_download_wait(setmhd_ctx* c, double* cell_par, double* t) {
+  if (!c || !cell_par) return SETMHD_ERR_ARG;
+  if (!c->snap_pending) return fail(c, SETMHD_ERR_ARG, "download_wait: no snapshot queued");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipEventSynchronize(c->ev_snap));
+  c->snap_pending = false;
+  std::memcpy(cell_par, c->snap_host, (size_t)c->snap_n * 16 * sizeof(double));
+  if (t) *t = c->snap_host[(size_t)c->snap_n * 16];
+  return SETMHD_OK;
 }
 
 int setmhd_probe(setmhd_ctx* c, int32_t n, const int32_t* cells, double* out, double* t) {

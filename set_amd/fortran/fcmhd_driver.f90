@@ -14,7 +14,9 @@
 ! are appended to 1D_rhoT.txt (write_T_rho) through setmhd_probe, one
 ! gather instead of a read per cell; with 'dump' every print_step writes
 ! output_data_<k>.txt (x, y, rho: Print_mult) and the end output_data.txt
-! (x, y, rho, u, v, p, |V|: Print).
+! (x, y, rho, u, v, p, |V|: Print).  A dump's state is taken with
+! setmhd_download_async and written at the next print, so its copy overlaps
+! the steps in between.
 !
 ! usage: fcmhd_driver input.bin output.bin all_step [print_step] [restart.bin|-]
 !                     [cartesian|reference] [probes.txt|-] [dump]
@@ -23,7 +25,7 @@ program fcmhd_driver
   use setmhd
   implicit none
 
-  real(8) :: host_time_all, vv, t, dt_next
+  real(8) :: host_time_all, vv, t, dt_next, tdump
   integer(4) :: host_N_cell, host_N_gran
   real(8), allocatable :: host_Cell_par(:, :), host_Cell_center(:, :), host_Cell_Volume(:)
   integer(4), allocatable :: host_Cell_gran(:, :)
@@ -161,12 +163,13 @@ program fcmhd_driver
       call append_probes(t, pv(1, :))
     end if
     if (dump) then
-      st = setmhd_download(ctx, host_Cell_par, t)
-      call setmhd_check(ctx, st, 'download')
+      if (k_dump > 0) call collect_dump()
+      st = setmhd_download_async(ctx)
+      call setmhd_check(ctx, st, 'download_async')
       k_dump = k_dump + 1
-      call dump_density(k_dump, t)
     end if
   end do
+  if (dump .and. k_dump > 0) call collect_dump()
 
   ! ---- Send_data_to_Host + Save_Storage --------------------------------------
   st = setmhd_download(ctx, host_Cell_par, host_time_all)
@@ -181,6 +184,13 @@ program fcmhd_driver
   if (dump) call dump_state()
 
 contains
+
+  ! writes dump k_dump from the snapshot queued at the previous print
+  subroutine collect_dump()
+    st = setmhd_download_wait(ctx, host_Cell_par, tdump)
+    call setmhd_check(ctx, st, 'download_wait')
+    call dump_density(k_dump, tdump)
+  end subroutine
 
   ! SET's dimensional units: the reference scales with default-real
   ! (single-precision) literals, so these are their single-precision values.

@@ -82,6 +82,18 @@ module setmhd
       real(c_double), intent(out) :: t
     end function
 
+    integer(c_int) function setmhd_download_async(ctx) bind(C, name="setmhd_download_async")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function setmhd_download_wait(ctx, cell_par, t) bind(C, name="setmhd_download_wait")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(inout) :: cell_par(*)
+      real(c_double), intent(out) :: t
+    end function
+
     integer(c_int) function setmhd_probe(ctx, n, cells, out, t) bind(C, name="setmhd_probe")
       import :: c_int, c_int32_t, c_double, c_ptr
       type(c_ptr), value :: ctx

@@ -37,7 +37,8 @@ EXPORTS = (
     "setmhd_config_reference", "setmhd_config_cartesian", "setmhd_create", "setmhd_destroy",
     "setmhd_upload_mesh", "setmhd_upload_structured", "setmhd_set_dt", "setmhd_dt_from_state",
     "setmhd_step", "setmhd_step_timed", "setmhd_sync", "setmhd_download",
-    "setmhd_download_potok", "setmhd_probe", "setmhd_sizes", "setmhd_residual", "setmhd_last_error",
+    "setmhd_download_potok", "setmhd_probe", "setmhd_download_async", "setmhd_download_wait",
+    "setmhd_sizes", "setmhd_residual", "setmhd_last_error",
     "setmhd_batch_add", "setmhd_batch_remove", "setmhd_batch_step", "setmhd_batch_step_async",
     "setmhd_batch_poll", "setmhd_batch_wait", "setmhd_batch_query",
     "setmhd_batch_download", "setmhd_batch_autotune", "setmhd_batch_set_launch", "setmhd_mem_info",
@@ -88,6 +89,8 @@ def load():
     L.setmhd_download.argtypes = [P, PD, PD]
     L.setmhd_download_potok.argtypes = [P, PD]
     L.setmhd_probe.argtypes = [P, I32, PI, PD, PD]
+    L.setmhd_download_async.argtypes = [P]
+    L.setmhd_download_wait.argtypes = [P, PD, PD]
     L.setmhd_sizes.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     L.setmhd_residual.argtypes = [P, PD]
     L.setmhd_last_error.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t]
@@ -240,6 +243,18 @@ class Context:
         p = np.zeros((self.n_gran, 9))
         self._check(self._L.setmhd_download_potok(self._h, _pd(p)))
         return p
+
+    def download_async(self):
+        """Queues a snapshot of Cell_par and the time behind the queued
+        steps; later steps may be queued before download_wait collects it."""
+        self._check(self._L.setmhd_download_async(self._h))
+
+    def download_wait(self):
+        """The snapshot queued by download_async: (Cell_par array, t)."""
+        cp = np.zeros((self.n_cell, 16))
+        t = ctypes.c_double()
+        self._check(self._L.setmhd_download_wait(self._h, _pd(cp), ctypes.byref(t)))
+        return cp, t.value
 
     def probe(self, cells):
         """Cell_par(1:8) of the given 1-based cell numbers from the current

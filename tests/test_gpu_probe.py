@@ -39,3 +39,35 @@ def test_probe_face_list_and_range():
             c.probe([fm.n_cell + 1])
     np.testing.assert_array_equal(pr, cp[cells - 1, :8])
     assert tp == t
+
+
+@pytest.mark.parametrize("path", ["structured", "face-list"])
+def test_download_async_snapshot(path):
+    """A snapshot holds the state at the point it was queued, although more
+    steps are queued before it is collected; one snapshot at a time."""
+    print(f"[case] download_async snapshot, {path}", flush=True)
+    if path == "structured":
+        g = mesh.grid_3d(40, 36, 20)
+        W = mesh.heliosphere_state(g, wind=mesh.Wind(r_in=0.3))
+        ctx = solver.Context(solver.cartesian_config())
+    else:
+        fm, W = workloads.heliosphere_face_mesh(60, 30)
+        ctx = solver.Context(solver.reference_config())
+    with ctx as c:
+        if path == "structured":
+            c.upload_structured(g, W)
+            c.dt_from_state()
+        else:
+            c.upload_mesh(fm, W)
+            c.set_dt(2.0e-9)
+        c.step(3)
+        ref, tref = c.download()
+        c.download_async()
+        with pytest.raises(solver.SetMHDError):
+            c.download_async()
+        c.step(4)  # queued behind the snapshot
+        snap, tsnap = c.download_wait()
+        after, tafter = c.download()
+    np.testing.assert_array_equal(snap, ref)
+    assert tsnap == tref and tafter > tref
+    assert not np.array_equal(after[:, :8], ref[:, :8])
