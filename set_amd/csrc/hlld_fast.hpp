@@ -50,11 +50,61 @@ __device__ __forceinline__ void phys_flux(const Side<R>& s, R F[8]) {
   F[7] = s.vn * s.bt2 - s.vt2 * s.bn;
 }
 
+// The default side accessor of hlld: pick(true) is the left state,
+// pick(false) the right one, chosen per lane by selects.  Callers whose
+// states sit in LDS pass an accessor that re-reads the chosen state instead
+// (a select costs one VALU instruction per 32 bits).
+template <class R>
+struct PickSelect {
+  const Side<R>& L;
+  const Side<R>& Rr;
+  __device__ __forceinline__ Side<R> operator()(bool left) const {
+    Side<R> s;
+    s.rho = left ? L.rho : Rr.rho;
+    s.vn = left ? L.vn : Rr.vn;
+    s.vt1 = left ? L.vt1 : Rr.vt1;
+    s.vt2 = left ? L.vt2 : Rr.vt2;
+    s.bn = left ? L.bn : Rr.bn;
+    s.bt1 = left ? L.bt1 : Rr.bt1;
+    s.bt2 = left ? L.bt2 : Rr.bt2;
+    s.pt = left ? L.pt : Rr.pt;
+    s.e = left ? L.e : Rr.e;
+    s.cf = left ? L.cf : Rr.cf;
+    return s;
+  }
+};
+
+// Tangential velocity and field of one side's single-star state
+// (:1624-1656): X is the side, m = rho (S - vn), su = S - vn, S its outer
+// wave speed.  The |tt| <= 1e-9 switch leaves the side's tangential state.
+template <class R>
+__device__ __forceinline__ void star_tangential(const Side<R>& X, R S, R m, R su, R SM, R bn, R bn2,
+                                                R& v1, R& v2, R& b1, R& b2) {
+  R tv = R(0), tb = R(0);
+  const R tt = m * (S - SM) - bn2;
+  if (fabs(tt) > R(1e-9)) {
+    const R it = frcp(tt);
+    tv = (SM - X.vn) * it;
+    tb = (m * su - bn2) * it;
+  }
+  v1 = X.vt1 - bn * X.bt1 * tv;
+  v2 = X.vt2 - bn * X.bt2 * tv;
+  b1 = X.bt1 * tb;
+  b2 = X.bt2 * tb;
+}
+
 // HLLD flux F (face frame) and max(|SL|,|SR|) for the CFL estimate.
 // Returns false where no wave region matches (NaN input; the reference
 // STOPs there, :1816-1825).
-template <class R>
-__device__ __forceinline__ bool hlld(const Side<R>& L, const Side<R>& Rr, R F[8], R& smax) {
+//
+// The regions are checked in the reference's order (:1722-1814, later test
+// wins); for any non-NaN state the winner lies left of the contact (regions
+// 2, 3) exactly when SM > 0, so the side whose fan holds the face is known
+// as soon as SM is: only that side's star state is formed, and the other
+// side's tangential star state only in the double-star regions 3 and 4.
+template <class R, class Pick>
+__device__ __forceinline__ bool hlld(const Side<R>& L, const Side<R>& Rr, R F[8], R& smax,
+                                     const Pick& pick) {
   const R SL = fmin(L.vn - L.cf, Rr.vn - Rr.cf);  // :1316-1319
   const R SR = fmax(L.vn + L.cf, Rr.vn + Rr.cf);
   const R aSL = fabs(SL), aSR = fabs(SR);
@@ -62,18 +112,7 @@ __device__ __forceinline__ bool hlld(const Side<R>& L, const Side<R>& Rr, R F[8]
   const R fbn = -(smax * (Rr.bn - L.bn));  // :1828-1839 (w = 0)
 
   if (SL > R(0) || SR < R(0)) {  // regions 1 and 6: upwind
-    const bool up = SL > R(0);
-    Side<R> U;
-    U.rho = up ? L.rho : Rr.rho;
-    U.vn = up ? L.vn : Rr.vn;
-    U.vt1 = up ? L.vt1 : Rr.vt1;
-    U.vt2 = up ? L.vt2 : Rr.vt2;
-    U.bn = up ? L.bn : Rr.bn;
-    U.bt1 = up ? L.bt1 : Rr.bt1;
-    U.bt2 = up ? L.bt2 : Rr.bt2;
-    U.pt = up ? L.pt : Rr.pt;
-    U.e = up ? L.e : Rr.e;
-    phys_flux(U, F);
+    phys_flux(pick(SL > R(0)), F);
     F[5] = fbn;
     return true;
   }
@@ -81,44 +120,18 @@ __device__ __forceinline__ bool hlld(const Side<R>& L, const Side<R>& Rr, R F[8]
   const R mR = suR * Rr.rho, mL = suL * L.rho;
   const R idn = frcp(mR - mL);
   const R SM = (mR * Rr.vn - Rr.pt + L.pt - mL * L.vn) * idn;            // :1343
+  const bool left = SM > R(0);
+  const Side<R> Q = pick(left);  // the side whose fan holds the face
   const R ptz = (mR * L.pt - mL * Rr.pt + mR * mL * (Rr.vn - L.vn)) * idn;  // :1600
   const R bn = (SR * Rr.bn - SL * L.bn) * frcp(SR - SL);                  // UZ(6)
   const R bn2 = bn * bn;
   const R idR = frcp(SR - SM), idL = frcp(SL - SM);
   const R rzR = mR * idR, rzL = mL * idL;  // r*su/(S-SM)
 
-  // single-star states (:1624-1656)
-  R tvL = R(0), tbL = R(0), tvR = R(0), tbR = R(0);
-  {
-    R tt = mL * (SL - SM) - bn2;
-    if (fabs(tt) > R(1e-9)) {
-      R it = frcp(tt);
-      tvL = (SM - L.vn) * it;
-      tbL = (mL * suL - bn2) * it;
-    }
-    tt = mR * (SR - SM) - bn2;
-    if (fabs(tt) > R(1e-9)) {
-      R it = frcp(tt);
-      tvR = (SM - Rr.vn) * it;
-      tbR = (mR * suR - bn2) * it;
-    }
-  }
-  const R vL1 = L.vt1 - bn * L.bt1 * tvL, vL2 = L.vt2 - bn * L.bt2 * tvL;
-  const R vR1 = Rr.vt1 - bn * Rr.bt1 * tvR, vR2 = Rr.vt2 - bn * Rr.bt2 * tvR;
-  const R bL1 = L.bt1 * tbL, bL2 = L.bt2 * tbL;
-  const R bR1 = Rr.bt1 * tbR, bR2 = Rr.bt2 * tbR;
-  const R sbvL = bn * SM + bL1 * vL1 + bL2 * vL2;
-  const R sbvR = bn * SM + bR1 * vR1 + bR2 * vR2;
-  const R vbL = L.vn * L.bn + L.vt1 * L.bt1 + L.vt2 * L.bt2;
-  const R vbR = Rr.vn * Rr.bn + Rr.vt1 * Rr.bt1 + Rr.vt2 * Rr.bt2;
-  const R ezL = L.e * (suL * idL) + (ptz * SM - L.pt * L.vn + bn * (vbL - sbvL)) * idL;
-  const R ezR = Rr.e * (suR * idR) + (ptz * SM - Rr.pt * Rr.vn + bn * (vbR - sbvR)) * idR;
-
-  // double-star states (:1658-1698)
+  // double-star speeds (:1658-1698)
   const R abn = fabs(bn);
   const bool ibn = abn > R(1e-6);
   const R irL = frsq(rzL), irR = frsq(rzR);  // 1/sqrt(rho*)
-  const R sqL = rzL * irL, sqR = rzR * irR;
   const R SZL = ibn ? SM - abn * irL : SM;
   const R SZR = ibn ? SM + abn * irR : SM;
 
@@ -128,24 +141,18 @@ __device__ __forceinline__ bool hlld(const Side<R>& L, const Side<R>& Rr, R F[8]
   if (ibn && SZL <= R(0) && SM >= R(0)) j = 3;
   if (ibn && SM <= R(0) && SZR >= R(0)) j = 4;
   if (SZR <= R(0)) j = 5;                   // SR >= 0 holds here
-  const bool left = j <= 3;
 
-  // star state of the chosen side and its HLL jump
+  // single-star state of the chosen side (:1624-1656) and its HLL jump
   const R S = left ? SL : SR;
+  const R su = left ? suL : suR;
+  const R m = left ? mL : mR;
+  const R id = left ? idL : idR;
   const R rz = left ? rzL : rzR;
-  const R vz1 = left ? vL1 : vR1, vz2 = left ? vL2 : vR2;
-  const R bz1 = left ? bL1 : bR1, bz2 = left ? bL2 : bR2;
-  const R ez = left ? ezL : ezR;
-  Side<R> Q;  // the side whose fan holds the face
-  Q.rho = left ? L.rho : Rr.rho;
-  Q.vn = left ? L.vn : Rr.vn;
-  Q.vt1 = left ? L.vt1 : Rr.vt1;
-  Q.vt2 = left ? L.vt2 : Rr.vt2;
-  Q.e = left ? L.e : Rr.e;
-  Q.bn = left ? L.bn : Rr.bn;
-  Q.bt1 = left ? L.bt1 : Rr.bt1;
-  Q.bt2 = left ? L.bt2 : Rr.bt2;
-  Q.pt = left ? L.pt : Rr.pt;
+  R vz1, vz2, bz1, bz2;
+  star_tangential(Q, S, m, su, SM, bn, bn2, vz1, vz2, bz1, bz2);
+  const R sbvz = bn * SM + bz1 * vz1 + bz2 * vz2;
+  const R vbQ = Q.vn * Q.bn + Q.vt1 * Q.bt1 + Q.vt2 * Q.bt2;
+  const R ez = Q.e * (su * id) + (ptz * SM - Q.pt * Q.vn + bn * (vbQ - sbvz)) * id;
   phys_flux(Q, F);  // F_L or F_R
   // F* = F_Q + S (U* - U_Q)
   F[0] += S * (rz - Q.rho);
@@ -157,19 +164,30 @@ __device__ __forceinline__ bool hlld(const Side<R>& L, const Side<R>& Rr, R F[8]
   F[7] += S * (bz2 - Q.bt2);
 
   if (j == 3 || j == 4) {  // U** - U*, weighted by SZL / SZR
+    // kept a branch (an empty volatile asm is not speculated): waves whose
+    // lanes all sit outside regions 3 and 4 skip it
+    asm volatile("");
+    // the other side O and its tangential single-star state
+    const Side<R> O = pick(!left);
+    const R So = left ? SR : SL;
+    const R suO = left ? suR : suL;
+    const R mO = left ? mR : mL;
+    R vo1, vo2, bo1, bo2;
+    star_tangential(O, So, mO, suO, SM, bn, bn2, vo1, vo2, bo1, bo2);
+    const R sqL = rzL * irL, sqR = rzR * irR;  // sqrt(rho*)
+    const R sq = left ? sqL : sqR, sqo = left ? sqR : sqL;
+    // sbn (b_R - b_L) = so (b_O - b_Q)
     const R sbn = bn > R(0) ? R(1) : R(-1);
+    const R so = left ? sbn : -sbn;
     const R iss = frcp(sqL + sqR);
-    const R w1 = (sqL * vL1 + sqR * vR1 + sbn * (bR1 - bL1)) * iss;
-    const R w2 = (sqL * vL2 + sqR * vR2 + sbn * (bR2 - bL2)) * iss;
-    const R c1 = (sqL * bR1 + sqR * bL1 + sbn * sqL * sqR * (vR1 - vL1)) * iss;
-    const R c2 = (sqL * bR2 + sqR * bL2 + sbn * sqL * sqR * (vR2 - vL2)) * iss;
+    const R w1 = (sq * vz1 + sqo * vo1 + so * (bo1 - bz1)) * iss;
+    const R w2 = (sq * vz2 + sqo * vo2 + so * (bo2 - bz2)) * iss;
+    const R c1 = (sq * bo1 + sqo * bz1 + so * sq * sqo * (vo1 - vz1)) * iss;
+    const R c2 = (sq * bo2 + sqo * bz2 + so * sq * sqo * (vo2 - vz2)) * iss;
     const R sbzz = bn * SM + c1 * w1 + c2 * w2;
-    const R sq = left ? sqL : sqR;
-    const R sbvz = left ? sbvL : sbvR;
-    const R sgn = left ? R(-1) : R(1);
     const R SZ = left ? SZL : SZR;
     // e** = e* -/+ sqrt(rho*) sgn(bn) (b*.v* - b**.v**)
-    const R dezz = sgn * sq * sbn * (sbvz - sbzz);
+    const R dezz = so * sq * (sbzz - sbvz);
     F[2] += SZ * (rz * (w1 - vz1));
     F[3] += SZ * (rz * (w2 - vz2));
     F[4] += SZ * dezz;
@@ -178,6 +196,11 @@ __device__ __forceinline__ bool hlld(const Side<R>& L, const Side<R>& Rr, R F[8]
   }
   F[5] = fbn;
   return j != 0;
+}
+
+template <class R>
+__device__ __forceinline__ bool hlld(const Side<R>& L, const Side<R>& Rr, R F[8], R& smax) {
+  return hlld(L, Rr, F, smax, PickSelect<R>{L, Rr});
 }
 
 }  // namespace setmhd

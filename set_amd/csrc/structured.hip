@@ -105,23 +105,40 @@ __device__ __forceinline__ void put_q(Smem<R, TX, TY>& sm, int ly, int lx, const
   sm.q[QCF + 1][ly][lx] = q.cf[1];
 }
 
-// face-frame side of the LDS cell (ly, lx) for an x- (D = 0) or y- (D = 1) face
+// face-frame side of the LDS cell at offset off = ly (TX + 2) + lx of each
+// face-data slot, for an x- (D = 0) or y- (D = 1) face
 template <int D, class R, int TX, int TY>
-__device__ __forceinline__ Side<R> get_side(const Smem<R, TX, TY>& sm, int ly, int lx) {
+__device__ __forceinline__ Side<R> get_side(const Smem<R, TX, TY>& sm, int off) {
   constexpr int D1 = D + 1, D2 = D == 0 ? 2 : 0;
+  auto at = [&](int f) { return (&sm.q[f][0][0])[off]; };
   Side<R> s;
-  s.rho = sm.q[QR][ly][lx];
-  s.vn = sm.q[QU + D][ly][lx];
-  s.vt1 = sm.q[QU + D1][ly][lx];
-  s.vt2 = sm.q[QU + D2][ly][lx];
-  s.bn = sm.q[QB + D][ly][lx];
-  s.bt1 = sm.q[QB + D1][ly][lx];
-  s.bt2 = sm.q[QB + D2][ly][lx];
-  s.pt = sm.q[QPT][ly][lx];
-  s.e = sm.q[QE][ly][lx];
-  s.cf = sm.q[QCF + D][ly][lx];
+  s.rho = at(QR);
+  s.vn = at(QU + D);
+  s.vt1 = at(QU + D1);
+  s.vt2 = at(QU + D2);
+  s.bn = at(QB + D);
+  s.bt1 = at(QB + D1);
+  s.bt2 = at(QB + D2);
+  s.pt = at(QPT);
+  s.e = at(QE);
+  s.cf = at(QCF + D);
   return s;
 }
+
+// hlld's side accessor for faces whose states sit in LDS: re-reads the
+// chosen state (ds_read) instead of selecting it from the two in registers.
+// The offset goes through an empty asm so that the compiler cannot fold the
+// reads back into selects of the earlier ones.
+template <int D, class R, int TX, int TY>
+struct PickLds {
+  const Smem<R, TX, TY>& sm;
+  int offL, offR;
+  __device__ __forceinline__ Side<R> operator()(bool left) const {
+    int off = left ? offL : offR;
+    asm("" : "+v"(off));
+    return get_side<D>(sm, off);
+  }
+};
 
 // One x- (D = 0) or y- (D = 1) face between LDS cells (ly, lx) and
 // (ry, rx); its flux, in x,y,z components, goes to the LDS flux slot of the
@@ -130,10 +147,11 @@ template <int D, class R, int TX, int TY>
 __device__ __forceinline__ void lds_face(Smem<R, TX, TY>& sm, int ly, int lx, int ry, int rx,
                                          double& smax, bool& ok) {
   constexpr int D1 = D + 1, D2 = D == 0 ? 2 : 0;
-  const Side<R> L = get_side<D>(sm, ly, lx);
-  const Side<R> Rr = get_side<D>(sm, ry, rx);
+  const int offL = ly * (TX + 2) + lx, offR = ry * (TX + 2) + rx;
+  const Side<R> L = get_side<D>(sm, offL);
+  const Side<R> Rr = get_side<D>(sm, offR);
   R F[8], s;
-  ok &= hlld(L, Rr, F, s);
+  ok &= hlld(L, Rr, F, s, PickLds<D, R, TX, TY>{sm, offL, offR});
   smax = fmax(smax, (double)s);
   auto& dst = [&]() -> auto& {
     if constexpr (D == 0) return sm.fx;
