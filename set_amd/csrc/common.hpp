@@ -73,7 +73,9 @@ __device__ __forceinline__ double fsqrt(double x) {
   double d = __builtin_fma(-g, g, x);
   return __builtin_fma(d, h, g);
 }
-__device__ __forceinline__ float fsqrt(float x) { return __builtin_sqrtf(x); }
+// FP32: the hardware instruction (1 ulp; sqrtf itself expands to a
+// correctly rounded sequence of about a dozen instructions).
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // 1/sqrt of a positive normal number: estimate + two Newton steps (FP64).
 __device__ __forceinline__ double frsq(double x) {
