@@ -227,7 +227,8 @@ int setmhd_batch_autotune(setmhd_ctx *ctx, int32_t probe_steps,
 /* Sets the batch launch of 2-D (dim 2), 3-D (dim 3) or all (dim 0) grids
  * without timing: tile shape (32x8, 16x16, 64x4 or 16x8; 3-D grids only,
  * 2-D grids keep 32x8) and blocks per CU (1..64 persistent blocks, or 0 for
- * one block per tile).  Defaults: 32x8 with one block per tile. */
+ * one block per tile).  Defaults: one block per tile; 32x8 tiles for the
+ * 3-D grids in FP64, 64x4 in the mixed form. */
 int setmhd_batch_set_launch(setmhd_ctx *ctx, int32_t dim, int32_t tile_x, int32_t tile_y,
                             int32_t blocks_per_cu);
 

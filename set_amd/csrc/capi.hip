@@ -536,6 +536,10 @@ int setmhd_create(setmhd_ctx** out, const setmhd_config* cfg) {
   c->cfg = *cfg;
   c->device = cfg->device;
   if (cfg->kchunk > 0) c->kchunk = cfg->kchunk;
+  if (cfg->precision == SETMHD_PREC_MIXED) {  // 3-D batch tiles: 64x4 in the mixed form
+    c->bt_x[1] = 64;                         // (12 serving-mix grids: +6 % against 32x8)
+    c->bt_y[1] = 4;
+  }
   auto bad = [&](int code) {
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;

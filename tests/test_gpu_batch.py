@@ -40,7 +40,7 @@ def grids():
 
 @pytest.mark.parametrize("precision,launch", [(0, ((32, 8), 2, 0)), (1, ((32, 8), 2, 0)),
                                               (0, ((32, 8), 0, 0)), (1, ((64, 4), 0, 0)),
-                                              (0, ((16, 16), 64, 0)), (0, None)])
+                                              (0, ((16, 16), 64, 0)), (0, None), (1, None)])
 def test_batch_equals_alone(precision, launch):
     case(f"batch of 5 (2-D + 3-D) vs alone, precision={precision}, launch={launch}")
     gs = grids()
