@@ -48,8 +48,10 @@ def from_profiles(name, key):
     return None
 
 
-def run_form(g, W, precision, time_scheme, steps, warmup):
-    from set_amd import solver
+def run_form(g, W, precision, time_scheme, steps, warmup, keep=False):
+    """One form on grid g from Cell_par W.  keep: also return the final
+    conserved state (key "_state", for the mixed form's error table)."""
+    from set_amd import mesh, solver
     cfg = solver.cartesian_config(cfl=0.4, r_hold=0.1, precision=precision,
                                   time_scheme=time_scheme)
     with solver.Context(cfg) as c:
@@ -68,8 +70,10 @@ def run_form(g, W, precision, time_scheme, steps, warmup):
         ms_ev = c.step_timed(steps)  # kernel time on the library's stream
         t_sim, dt = c.sync()
         t0 = time.perf_counter()
-        c.download()
+        cp, _ = c.download()
         t_down = time.perf_counter() - t0
+    state = mesh.prim_to_cons(cp[:, :8]) if keep else None
+    del cp
     launches = steps * (2 if time_scheme == 1 else 1)
     bytes_per_cell = 320 if time_scheme == 1 else 128
     return {
@@ -84,7 +88,21 @@ def run_form(g, W, precision, time_scheme, steps, warmup):
         "pcie_inclusive_value": g.n_cell * steps / (wall + t_up + t_down),
         "t_sim": t_sim,
         "dt": dt,
+        "steps_run": warmup + 2 * steps,
+        "_state": state,
     }
+
+
+def error_table(a, b):
+    """Per-variable normwise relative L1 and L-inf errors of conserved state a
+    against b: sum|a - b| / sum|b| and max|a - b| / max|b| per variable."""
+    import numpy as np
+    names = ("rho", "mx", "my", "mz", "E", "bx", "by", "bz")
+    d = np.abs(a - b)
+    l1 = d.sum(0) / np.maximum(np.abs(b).sum(0), 1e-300)
+    li = d.max(0) / np.maximum(np.abs(b).max(0), 1e-300)
+    return {"rel_l1": {n: float(v) for n, v in zip(names, l1)},
+            "rel_linf": {n: float(v) for n, v in zip(names, li)}}
 
 
 def batch_form(precision, steps, seq_steps):
@@ -232,14 +250,18 @@ def main():
     W = mesh.heliosphere_state(g)
     forms = {}
     wanted = args.forms.split(",")
-    fp = run_form(g, W, 0, 0, args.steps, args.warmup)
+    fp = run_form(g, W, 0, 0, args.steps, args.warmup, keep="mixed" in wanted)
     forms["fp64_euler"] = fp
     if "mixed" in wanted:
-        forms["mixed_euler"] = run_form(g, W, 1, 0, args.steps, args.warmup)
-        forms["mixed_euler"]["speedup_vs_fp64"] = forms["mixed_euler"]["value"] / fp["value"]
+        mx = run_form(g, W, 1, 0, args.steps, args.warmup, keep=True)
+        forms["mixed_euler"] = mx
+        mx["speedup_vs_fp64"] = mx["value"] / fp["value"]
+        # the precision form's error against FP64 after the same steps
+        mx["error_vs_fp64"] = {"steps": mx["steps_run"], **error_table(mx["_state"], fp["_state"])}
     if "heun" in wanted:
         forms["fp64_heun"] = run_form(g, W, 0, 1, args.steps, args.warmup)
     for f in forms.values():
+        f.pop("_state", None)
         f["hbm_frac"] = f["achieved_GBs"] / PEAK_HBM_GBS
     del W
     for name, fn in (("batch64_fp64", lambda: batch_form(0, args.batch_steps, 20)),
