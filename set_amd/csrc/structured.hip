@@ -228,15 +228,6 @@ __device__ __forceinline__ void finalize_with(const FusedArgs& a, double dt, uns
   for (int v = 0; v < 8; v++) (a.Uo + v * g.nc)[idx] = Un[v];
 }
 
-// finalize_with, loading U^n of cell idx here
-template <int MODE>
-__device__ __forceinline__ void finalize(const FusedArgs& a, double dt, unsigned idx, bool held,
-                                         const double acc[8], double& rs_d, double& rs_r) {
-  double Uold[8];
-  load8(a.Uf, a.g.nc, idx, Uold);
-  finalize_with<MODE>(a, dt, idx, held, acc, Uold, rs_d, rs_r);
-}
-
 // Raw FP64 state of the own cells of one plane of a tile, [field][row][x],
 // filled by LDS DMA (global_load_lds).
 template <int TX, int TY>
