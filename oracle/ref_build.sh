@@ -6,9 +6,11 @@
 #   Solvers.f90:1143-1859         chlld
 # Solvers.f90 as a whole cannot compile here: it needs the CUDA Fortran
 # module cudafor.  The two routine ranges need nothing but STORAGE, so the
-# recipe cuts them out (sed, unmodified text) into a module in _ref/.
+# recipe cuts them out (sed, unmodified text) into a module source in _ref/.
 # The '!@cuf attributes(host, device)' lines are comments for a compiler
-# without CUDA Fortran.  Output goes only to oracle/_ref/ (git-ignored).
+# without CUDA Fortran.  The cut-out module source and the compiler's
+# module and object files are deleted once the library is linked, so
+# oracle/_ref/ (git-ignored) ends up holding only libref_chlld.so.
 set -euo pipefail
 HERE="$(cd "$(dirname "$0")" && pwd)"
 REF="${SET_REFERENCE:-/root/reference}/SET_Fortran_Cuda_MHD"
@@ -33,4 +35,5 @@ FFLAGS="-O2 -fPIC -ffp-contract=off"
 "$FC" $FFLAGS -c ref_chlld_mod.f90 -o ref_chlld_mod.o
 "$FC" $FFLAGS -c "$HERE/ref_bind.f90" -o ref_bind.o
 "$FC" -shared -fPIC -o libref_chlld.so storage.o ref_chlld_mod.o ref_bind.o
+rm -f ref_chlld_mod.f90 storage.mod ref_chlld_mod.mod ref_bind.mod storage.o ref_chlld_mod.o ref_bind.o
 echo "built $OUT/libref_chlld.so"
