@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
+from conftest import rel_err
 from set_amd import mesh, solver, workloads
 
 pytestmark = pytest.mark.gpu
@@ -121,6 +122,34 @@ def test_fast_build_agrees_with_oracle(which):
     print("rel Linf per var", err)
     assert np.all(err <= 1e-12), err
     assert t == pytest.approx(s.t, rel=1e-13)
+
+
+@pytest.mark.parametrize("nsteps,bound,order", [(1, 1e-12, "inf"), (500, 1e-9, "l1")])
+def test_fast_build_hlld_plain_2d(nsteps, bound, order):
+    """SETMHD_F_FAST with HLLD and B != 0 (chlld with FMA contraction and
+    reciprocal-based division) on the plain 400x200 set-up against the
+    oracle: relative L-inf <= 1e-12 after one step, relative L1 <= 1e-9 on
+    all 8 conserved variables after 500, the bounds of the exact path."""
+    case(f"dropin fast build HLLD 400x200 steps={nsteps}")
+    g = mesh.grid_2d(400, 200)
+    fm = mesh.face_mesh(g)
+    W = mesh.heliosphere_state(g)
+    ocfg = oracle.cartesian_config()
+    dt0 = oracle.Solver(fm, W, ocfg).dt_scan()
+    s = oracle.Solver(fm, W, ocfg)
+    s.dt_next = dt0
+    assert s.run(nsteps) == 0
+    cfg = gpu_cfg(ocfg)
+    cfg.flags |= solver.F_FAST
+    with solver.Context(cfg) as c:
+        c.upload_mesh(fm, W)
+        c.set_dt(dt0)
+        c.step(nsteps)
+        cp, t = c.download()
+    e = rel_err(mesh.prim_to_cons(cp[:, :8]), mesh.prim_to_cons(s.W[:, :8]), order)
+    print("rel err per var", order, e)
+    assert np.all(e <= bound), e
+    assert t == pytest.approx(s.t, rel=1e-12)
 
 
 def test_density_failure_reported():
