@@ -11,9 +11,9 @@ cell-updates/s over the timed steps (wall clock between device syncs, state
 resident in HBM).  The roofline object uses the kernel's own duration from
 HIP events on the library's stream and 128 algorithmic bytes per
 cell-update (read U^n, write U^{n+1}: 8 FP64 each).  The mixed-precision
-and Heun-RK2 forms, the 64-grid batch, the serving stream and the drop-in
-face-list path at reference semantics are reported beside it under
-"forms".  cpu_baseline is
+and Heun-RK2 forms, the 64-grid batch, the serving stream, the drop-in
+face-list path at reference semantics, and the face-list path on this same
+box are reported beside it under "forms".  cpu_baseline is
 the CPU oracle (the C restatement of the reference algorithm) on a bounded
 sample of the same workload.
 
@@ -171,6 +171,32 @@ def dropin_form(nx, ny, steps):
     return out
 
 
+def facelist_form(n, steps):
+    """The drop-in face-list kernels (the reference's face pass and cell pass
+    restated, with chlld's HLLD) on the headline workload itself: the n^3
+    heliosphere box expressed as SET's face-list arrays, forward Euler, the
+    same dt rule and held wind cells; exact build (bit for bit with the
+    oracle) and fast build.  Beside the fused structured kernel this shows
+    what the Cartesian specialisation buys on the same cells."""
+    from set_amd import mesh, solver
+    g = mesh.grid_3d(n, n, n)
+    fm = mesh.face_mesh(g)
+    W = mesh.heliosphere_state(g)
+    out = {"mesh": f"{n}^3 box [-1,1]^3 as a face list", "cells": fm.n_cell, "faces": fm.n_gran,
+           "steps": steps, "time_scheme": "euler"}
+    for name, flags in (("exact", 0), ("fast", solver.F_FAST)):
+        cfg = solver.cartesian_config(cfl=0.4, r_hold=0.1)
+        cfg.flags |= flags
+        with solver.Context(cfg) as c:
+            c.upload_mesh(fm, W)
+            c.dt_from_state()
+            c.step(2)
+            c.sync()
+            ms = c.step_timed(steps) / steps
+        out[name] = {"value": fm.n_cell / (ms * 1e-3), "unit": "cell-updates/s", "ms_per_step": ms}
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -217,7 +243,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--forms", default="fp64,mixed,heun,batch64,serving,dropin")
+    ap.add_argument("--forms", default="fp64,mixed,heun,batch64,serving,dropin,facelist")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=4)
@@ -268,12 +294,17 @@ def main():
                      ("batch64_mixed", lambda: batch_form(1, args.batch_steps, 20)),
                      ("serving_fp64", lambda: serving_form(args.serve_requests, args.serve_max_steps)),
                      ("dropin_reference", lambda: dropin_form(*map(int, args.dropin_mesh.split("x")),
-                                                              args.dropin_steps))):
+                                                              args.dropin_steps)),
+                     ("facelist_headline", lambda: facelist_form(n, args.dropin_steps))):
         if name.split("_")[0] in wanted:
             try:
                 forms[name] = fn()
             except Exception as e:  # a failing extra form must not lose the headline line
                 forms[name] = {"error": repr(e)}
+    flf = forms.get("facelist_headline", {})
+    if "exact" in flf:
+        flf["fused_speedup_vs_exact"] = fp["value"] / flf["exact"]["value"]
+        flf["fused_speedup_vs_fast"] = fp["value"] / flf["fast"]["value"]
 
     traffic = from_profiles(f"fused_fp64_euler_{n}cube", "traffic_bytes_per_launch")
     achieved = fp["achieved_GBs"]
