@@ -154,15 +154,24 @@ def dropin_form(nx, ny, steps):
     LAX wedge, minmod TVD, Heun, axisymmetric and charge-exchange sources,
     w = B = 0, reference initial dt 2e-9) on a half-plane heliosphere mesh:
     the exact build (bit for bit with the oracle) and the fast build."""
-    from set_amd import solver, workloads
+    import numpy as np
+
+    from set_amd import mesh, solver, workloads
     fm, W = workloads.heliosphere_face_mesh(nx, ny)
+    # the same mesh numbered at random (a mesh file in no useful order), run
+    # as given and with SETMHD_F_REORDER (device numbering by Morton order)
+    rng = np.random.default_rng(1)
+    cp, gp = rng.permutation(fm.n_cell), rng.permutation(fm.n_gran)
+    fr = mesh.renumber_face_mesh(fm, cp, gp)
     out = {"mesh": f"{nx}x{ny} half-plane heliosphere face list", "cells": fm.n_cell,
            "faces": fm.n_gran, "steps": steps, "time_scheme": "heun"}
-    for name, flags in (("exact", 0), ("fast", solver.F_FAST)):
+    for name, m, w, flags in (("exact", fm, W, 0), ("fast", fm, W, solver.F_FAST),
+                              ("exact_random_numbering", fr, W[cp], 0),
+                              ("exact_random_numbering_reorder", fr, W[cp], solver.F_REORDER)):
         cfg = solver.reference_config()
         cfg.flags |= flags
         with solver.Context(cfg) as c:
-            c.upload_mesh(fm, W)
+            c.upload_mesh(m, w)
             c.set_dt(2.0e-9)
             c.step(3)
             c.sync()

@@ -54,6 +54,10 @@ enum setmhd_status {
 #define SETMHD_F_HELIO_HACKS 8 /* LAX wedge :296-298, u<=-0.1 clamp :320-322, v clamp :783-785 */
 #define SETMHD_F_FAST 16       /* drop-in kernels with FMA contraction and reciprocal-based
                                   division: same algorithm, agrees to rounding, not bit for bit */
+#define SETMHD_F_REORDER 32    /* face-list meshes: the device holds the cells in Morton order
+                                  of their centres and the faces by their first cell
+                                  (setmhd_mesh_order); every call still takes and returns the
+                                  caller's numbering.  Results are bit for bit the same. */
 
 #define SETMHD_RIEMANN_LAX 0
 #define SETMHD_RIEMANN_HLL 1
@@ -119,6 +123,18 @@ int setmhd_upload_mesh(setmhd_ctx *ctx, int32_t n_cell, int32_t n_gran,
                        const int32_t *gran_type /*[n_gran]*/,
                        const double *gran_potok /*[9*n_gran] or NULL*/,
                        double t0);
+
+/* The numbering SETMHD_F_REORDER gives a face-list mesh on the device, for
+ * callers that renumber their own files: cell_perm[i] (0-based) is the
+ * caller's cell placed i-th, cells sorted by the Morton (Z-order) key of
+ * their centre in the mesh's bounding box, 21 bits per axis, ties by
+ * number; gran_perm[f] is the caller's face placed f-th, faces sorted by
+ * the new place of their first cell (gran_neighbour(1,:)), ties by number.
+ * Host only (no device call).  No counterpart in the reference, whose
+ * kernels take the mesh in the order Setka.cpp:1158-1567 writes it. */
+int setmhd_mesh_order(int32_t n_cell, int32_t n_gran, const double *cell_center /*[3*n_cell]*/,
+                      const int32_t *gran_neighbour /*[2*n_gran]*/,
+                      int32_t *cell_perm /*[n_cell]*/, int32_t *gran_perm /*[n_gran]*/);
 
 /* Structured Cartesian box nx*ny*nz (nz = 1: 2-D) on
  * [box0,box1]x[box2,box3]x[box4,box5], outflow (copy) boundaries on every

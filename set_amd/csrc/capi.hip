@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -41,6 +42,10 @@ struct setmhd_ctx {
   UMesh um{};
   double *d_W = nullptr, *d_W2 = nullptr, *d_H = nullptr, *d_potok = nullptr;
   void* d_mesh = nullptr;
+  // SETMHD_F_REORDER: device place -> caller's cell / face (0-based), and
+  // caller's cell -> device place; empty when the caller's order is kept
+  std::vector<int32_t> cperm, fperm, cinv;
+  const int* d_cperm = nullptr;  // cperm on the device (inside d_mesh)
   // structured
   SGrid sg{};
   double *d_U = nullptr, *d_U2 = nullptr;  // Euler: ping-pong; Heun: U^n, U^1
@@ -190,6 +195,10 @@ void free_state(setmhd_ctx* c) {
   }
   if (c->d_mesh) hipFree(c->d_mesh);
   c->d_mesh = nullptr;
+  c->d_cperm = nullptr;
+  c->cperm.clear();
+  c->fperm.clear();
+  c->cinv.clear();
   free_batch(c);
   c->kind = 0;
 }
@@ -270,24 +279,28 @@ __global__ void k_probe(const double* __restrict__ W, const double* __restrict__
 }
 
 // AoS16 <-> AoS8 column split for the unstructured state
-__global__ void k_split16(const double* __restrict__ cp, long long n, double* __restrict__ W,
-                          double* __restrict__ H) {
+// Cell_par (caller's numbering) <-> W, H (device numbering): device cell c
+// is the caller's cell perm[c], or c when perm is null.
+__global__ void k_split16(const double* __restrict__ cp, long long n, const int* __restrict__ perm,
+                          double* __restrict__ W, double* __restrict__ H) {
   long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
+  const long long o = perm ? perm[c] : c;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    W[8 * c + k] = cp[16 * c + k];
-    H[8 * c + k] = cp[16 * c + 8 + k];
+    W[8 * c + k] = cp[16 * o + k];
+    H[8 * c + k] = cp[16 * o + 8 + k];
   }
 }
 __global__ void k_join16(const double* __restrict__ W, const double* __restrict__ H, long long n,
-                         double* __restrict__ cp) {
+                         const int* __restrict__ perm, double* __restrict__ cp) {
   long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
+  const long long o = perm ? perm[c] : c;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    cp[16 * c + k] = W[8 * c + k];
-    cp[16 * c + 8 + k] = H[8 * c + k];
+    cp[16 * o + k] = W[8 * c + k];
+    cp[16 * o + 8 + k] = H[8 * c + k];
   }
 }
 
@@ -463,6 +476,51 @@ int structured_step(setmhd_ctx* c, double* resid) {
   return SETMHD_OK;
 }
 
+// Spreads the low 21 bits of v to every third bit (Morton interleave).
+uint64_t spread3(uint64_t v) {
+  v &= 0x1fffffULL;
+  v = (v | v << 32) & 0x1f00000000ffffULL;
+  v = (v | v << 16) & 0x1f0000ff0000ffULL;
+  v = (v | v << 8) & 0x100f00f00f00f00fULL;
+  v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+  v = (v | v << 2) & 0x1249249249249249ULL;
+  return v;
+}
+
+// Device numbering of a face-list mesh (setmhd_mesh_order): cells by the
+// Morton key of their centre, faces by the new place of their first cell.
+// gran_neighbour(1,:) must lie in 1..n_cell.
+void mesh_order(int32_t nc, int32_t nf, const double* cc, const int32_t* nb, int32_t* cperm,
+                int32_t* fperm) {
+  double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+  for (int32_t i = 0; i < nc; i++)
+    for (int d = 0; d < 3; d++) {
+      lo[d] = std::min(lo[d], cc[3 * (size_t)i + d]);
+      hi[d] = std::max(hi[d], cc[3 * (size_t)i + d]);
+    }
+  const double top = (double)((1 << 21) - 1);
+  std::vector<uint64_t> key(nc);
+  for (int32_t i = 0; i < nc; i++) {
+    uint64_t k = 0;
+    for (int d = 0; d < 3; d++) {
+      double w = hi[d] > lo[d] ? (cc[3 * (size_t)i + d] - lo[d]) / (hi[d] - lo[d]) : 0.0;
+      if (!(w >= 0.0)) w = 0.0;  // also NaN
+      if (w > 1.0) w = 1.0;
+      k |= spread3((uint64_t)(w * top)) << d;
+    }
+    key[i] = k;
+  }
+  for (int32_t i = 0; i < nc; i++) cperm[i] = i;
+  std::sort(cperm, cperm + nc, [&](int32_t a, int32_t b) { return key[a] != key[b] ? key[a] < key[b] : a < b; });
+  std::vector<int32_t> cinv(nc);
+  for (int32_t i = 0; i < nc; i++) cinv[cperm[i]] = i;
+  for (int32_t f = 0; f < nf; f++) fperm[f] = f;
+  std::sort(fperm, fperm + nf, [&](int32_t a, int32_t b) {
+    const int32_t ca = cinv[nb[2 * (size_t)a] - 1], cb = cinv[nb[2 * (size_t)b] - 1];
+    return ca != cb ? ca < cb : a < b;
+  });
+}
+
 int unstructured_step(setmhd_ctx* c) {
   const UCfg k = ucfg(c->cfg);
   const int cur = (int)(c->step % 3);
@@ -582,6 +640,16 @@ void setmhd_destroy(setmhd_ctx* c) {
   delete c;
 }
 
+int setmhd_mesh_order(int32_t n_cell, int32_t n_gran, const double* cell_center,
+                      const int32_t* gran_neighbour, int32_t* cell_perm, int32_t* gran_perm) {
+  if (n_cell <= 0 || n_gran <= 0 || !cell_center || !gran_neighbour || !cell_perm || !gran_perm)
+    return SETMHD_ERR_ARG;
+  for (int32_t f = 0; f < n_gran; f++)
+    if (gran_neighbour[2 * (size_t)f] < 1 || gran_neighbour[2 * (size_t)f] > n_cell) return SETMHD_ERR_ARG;
+  mesh_order(n_cell, n_gran, cell_center, gran_neighbour, cell_perm, gran_perm);
+  return SETMHD_OK;
+}
+
 int setmhd_upload_mesh(setmhd_ctx* c, int32_t n_cell, int32_t n_gran, const double* cell_par,
                        const double* cell_center, const double* cell_volume,
                        const int32_t* cell_gran, const double* gran_normal,
@@ -614,8 +682,64 @@ int setmhd_upload_mesh(setmhd_ctx* c, int32_t n_cell, int32_t n_gran, const doub
   HIPCHK(c, hipStreamSynchronize(c->stream));
   free_state(c);
   const size_t nc = n_cell, nf = n_gran;
-  // one block for the geometry
-  size_t bytes = nc * (3 + 1) * 8 + nf * (3 + 1 + 3) * 8 + nc * 6 * 4 + nf * (2 + 2 + 1) * 4 + 64;
+  // SETMHD_F_REORDER: renumber the geometry on the host; the state is
+  // permuted on the device (k_split16 / k_join16)
+  const bool reorder = (c->cfg.flags & SETMHD_F_REORDER) != 0;
+  std::vector<int32_t> cperm, fperm, cinv, finv, r_cg, r_nb, r_tvd, r_ty;
+  std::vector<double> r_cc, r_cv, r_gn, r_gs, r_gc;
+  if (reorder) {
+    cperm.resize(nc);
+    fperm.resize(nf);
+    mesh_order(n_cell, n_gran, cell_center, gran_neighbour, cperm.data(), fperm.data());
+    cinv.resize(nc);
+    finv.resize(nf);
+    for (size_t i = 0; i < nc; i++) cinv[cperm[i]] = (int32_t)i;
+    for (size_t f = 0; f < nf; f++) finv[fperm[f]] = (int32_t)f;
+    auto cmap = [&](int32_t v) { return v >= 1 ? cinv[v - 1] + 1 : v; };  // -1 / 0 stay
+    r_cc.resize(nc * 3);
+    r_cv.resize(nc);
+    r_cg.resize(nc * 6);
+    for (size_t i = 0; i < nc; i++) {
+      const size_t o = cperm[i];
+      for (int d = 0; d < 3; d++) r_cc[3 * i + d] = cell_center[3 * o + d];
+      r_cv[i] = cell_volume[o];
+      for (int k = 0; k < 6; k++) {
+        const int32_t g = cell_gran[6 * o + k];
+        r_cg[6 * i + k] = g >= 1 ? finv[g - 1] + 1 : g;
+      }
+    }
+    r_gn.resize(nf * 3);
+    r_gs.resize(nf);
+    r_gc.resize(nf * 3);
+    r_nb.resize(nf * 2);
+    r_tvd.resize(nf * 2);
+    r_ty.resize(nf);
+    for (size_t f = 0; f < nf; f++) {
+      const size_t o = fperm[f];
+      for (int d = 0; d < 3; d++) {
+        r_gn[3 * f + d] = gran_normal[3 * o + d];
+        r_gc[3 * f + d] = gran_center[3 * o + d];
+      }
+      r_gs[f] = gran_square[o];
+      for (int k = 0; k < 2; k++) {
+        r_nb[2 * f + k] = cmap(gran_neighbour[2 * o + k]);
+        r_tvd[2 * f + k] = cmap(gran_neighbour_tvd[2 * o + k]);
+      }
+      r_ty[f] = gran_type[o];
+    }
+    cell_center = r_cc.data();
+    cell_volume = r_cv.data();
+    cell_gran = r_cg.data();
+    gran_normal = r_gn.data();
+    gran_square = r_gs.data();
+    gran_center = r_gc.data();
+    gran_neighbour = r_nb.data();
+    gran_neighbour_tvd = r_tvd.data();
+    gran_type = r_ty.data();
+  }
+  // one block for the geometry (and the cell permutation)
+  size_t bytes = nc * (3 + 1) * 8 + nf * (3 + 1 + 3) * 8 + nc * 6 * 4 + nf * (2 + 2 + 1) * 4 + 64 +
+                 (reorder ? nc * 4 + 8 : 0);
   char* base = nullptr;
   if (hipMalloc((void**)&base, bytes) != hipSuccess) return fail(c, SETMHD_ERR_NOMEM, "hipMalloc mesh");
   c->d_mesh = base;
@@ -634,6 +758,7 @@ int setmhd_upload_mesh(setmhd_ctx* c, int32_t n_cell, int32_t n_gran, const doub
   int* gnb = (int*)take(nf * 2 * 4);
   int* gtv = (int*)take(nf * 2 * 4);
   int* gty = (int*)take(nf * 4);
+  int* cpm = reorder ? (int*)take(nc * 4) : nullptr;
   HIPCHK(c, hipMemcpyAsync(cc, cell_center, nc * 3 * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(cv, cell_volume, nc * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(gn, gran_normal, nf * 3 * 8, hipMemcpyHostToDevice, c->stream));
@@ -643,6 +768,8 @@ int setmhd_upload_mesh(setmhd_ctx* c, int32_t n_cell, int32_t n_gran, const doub
   HIPCHK(c, hipMemcpyAsync(gnb, gran_neighbour, nf * 2 * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(gtv, gran_neighbour_tvd, nf * 2 * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(gty, gran_type, nf * 4, hipMemcpyHostToDevice, c->stream));
+  if (reorder) HIPCHK(c, hipMemcpyAsync(cpm, cperm.data(), nc * 4, hipMemcpyHostToDevice, c->stream));
+  c->d_cperm = cpm;
   c->um = UMesh{n_cell, n_gran, cc, cv, cg, gn, gs, gc, gnb, gtv, gty};
   int st;
   if ((st = dalloc(c, &c->d_W, nc * 8)) || (st = dalloc(c, &c->d_W2, nc * 8)) ||
@@ -652,13 +779,15 @@ int setmhd_upload_mesh(setmhd_ctx* c, int32_t n_cell, int32_t n_gran, const doub
   if ((st = dalloc(c, &stage, nc * 16))) return st;
   HIPCHK(c, hipMemcpyAsync(stage, cell_par, nc * 16 * 8, hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_split16, dim3(nblocks(nc)), dim3(256), 0, c->stream, stage, (long long)nc,
-                     c->d_W, c->d_H);
+                     c->d_cperm, c->d_W, c->d_H);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(c->d_W2, c->d_W, nc * 8 * 8, hipMemcpyDeviceToDevice, c->stream));  // :361
   if (gran_potok) {
     std::vector<double> p8(nf * 8);
-    for (size_t f = 0; f < nf; f++)
-      for (int k = 0; k < 8; k++) p8[8 * f + k] = gran_potok[9 * f + k];
+    for (size_t f = 0; f < nf; f++) {
+      const size_t o = reorder ? (size_t)fperm[f] : f;
+      for (int k = 0; k < 8; k++) p8[8 * f + k] = gran_potok[9 * o + k];
+    }
     HIPCHK(c, hipMemcpyAsync(c->d_potok, p8.data(), nf * 8 * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
   } else {
@@ -668,6 +797,9 @@ int setmhd_upload_mesh(setmhd_ctx* c, int32_t n_cell, int32_t n_gran, const doub
   hipFree(stage);
   c->n_cell = n_cell;
   c->n_gran = n_gran;
+  c->cperm = std::move(cperm);
+  c->fperm = std::move(fperm);
+  c->cinv = std::move(cinv);
   c->kind = 1;
   return upload_scalars(c, t0);
 }
@@ -843,7 +975,8 @@ namespace {
 hipError_t convert_state(setmhd_ctx* c, double* dst) {
   const long long nc = c->n_cell;
   if (c->kind == 1) {
-    hipLaunchKernelGGL(k_join16, dim3(nblocks(nc)), dim3(256), 0, c->stream, c->d_W, c->d_H, nc, dst);
+    hipLaunchKernelGGL(k_join16, dim3(nblocks(nc)), dim3(256), 0, c->stream, c->d_W, c->d_H, nc, c->d_cperm,
+                       dst);
   } else {
     const double* U = (c->cfg.time_scheme == SETMHD_TIME_EULER && c->cur_buf == 1) ? c->d_U2 : c->d_U;
     hipLaunchKernelGGL(k_cons_to_prim, dim3(nblocks(nc)), dim3(256), 0, c->stream, U, nc, dst);
@@ -910,7 +1043,7 @@ int setmhd_probe(setmhd_ctx* c, int32_t n, const int32_t* cells, double* out, do
   std::vector<int> ids(n);
   for (int32_t i = 0; i < n; i++) {
     if (cells[i] < 1 || cells[i] > c->n_cell) return fail(c, SETMHD_ERR_ARG, "probe: cell number out of range");
-    ids[i] = cells[i] - 1;
+    ids[i] = c->cinv.empty() ? cells[i] - 1 : c->cinv[cells[i] - 1];
   }
   HIPCHK(c, hipSetDevice(c->device));
   if (n > c->probe_cap) {  // grow-only: ids, then 8 doubles per cell
@@ -943,8 +1076,9 @@ int setmhd_download_potok(setmhd_ctx* c, double* gran_potok) {
   HIPCHK(c, hipMemcpyAsync(p8.data(), c->d_potok, p8.size() * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (long long f = 0; f < c->n_gran; f++) {
-    for (int k = 0; k < 8; k++) gran_potok[9 * f + k] = p8[8 * f + k];
-    gran_potok[9 * f + 8] = 0.0;  // cuf_solvers.cuf:433
+    const long long o = c->fperm.empty() ? f : c->fperm[f];  // the caller's face
+    for (int k = 0; k < 8; k++) gran_potok[9 * o + k] = p8[8 * f + k];
+    gran_potok[9 * o + 8] = 0.0;  // cuf_solvers.cuf:433
   }
   return SETMHD_OK;
 }

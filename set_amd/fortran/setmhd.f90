@@ -13,6 +13,7 @@ module setmhd
   integer(c_int), parameter :: SETMHD_OK = 0
   integer(c_int), parameter :: SETMHD_F_ZERO_BW = 1, SETMHD_F_AXISYM = 2
   integer(c_int), parameter :: SETMHD_F_SOURCE_MK = 4, SETMHD_F_HELIO_HACKS = 8
+  integer(c_int), parameter :: SETMHD_F_FAST = 16, SETMHD_F_REORDER = 32
 
   type, bind(C) :: setmhd_config
     integer(c_int32_t) :: riemann, recon, time_scheme, flags
@@ -55,6 +56,15 @@ module setmhd
       integer(c_int32_t), intent(in) :: gran_neighbour(*), gran_neighbour_tvd(*), gran_type(*)
       real(c_double), intent(in) :: gran_potok(*)
       real(c_double), value :: t0
+    end function
+
+    integer(c_int) function setmhd_mesh_order(n_cell, n_gran, cell_center, gran_neighbour, &
+        cell_perm, gran_perm) bind(C, name="setmhd_mesh_order")
+      import :: c_int, c_int32_t, c_double
+      integer(c_int32_t), value :: n_cell, n_gran
+      real(c_double), intent(in) :: cell_center(*)
+      integer(c_int32_t), intent(in) :: gran_neighbour(*)
+      integer(c_int32_t), intent(out) :: cell_perm(*), gran_perm(*)
     end function
 
     integer(c_int) function setmhd_set_dt(ctx, dt) bind(C, name="setmhd_set_dt")

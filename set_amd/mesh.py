@@ -172,6 +172,35 @@ def face_mesh(g: Grid, boundary_type=TYPE_OUTER) -> FaceMesh:
         gran_type=gran_type)
 
 
+def renumber_face_mesh(fm: FaceMesh, cell_perm, gran_perm) -> FaceMesh:
+    """fm with its cells and faces renumbered: new cell i is old cell
+    cell_perm[i], new face f is old face gran_perm[f] (0-based).  Cell and
+    face numbers in the connectivity are remapped, the -1 / 0 sentinels
+    kept.  A state goes along as cell_par[cell_perm], a face array as
+    gran_potok[gran_perm]."""
+    cp = np.asarray(cell_perm, np.int64)
+    gp = np.asarray(gran_perm, np.int64)
+    cinv = np.empty_like(cp)
+    cinv[cp] = np.arange(cp.size)
+    finv = np.empty_like(gp)
+    finv[gp] = np.arange(gp.size)
+
+    def remap(a, inv):
+        return np.where(a >= 1, inv[np.maximum(a, 1) - 1] + 1, a).astype(np.int32)
+
+    return FaceMesh(
+        n_cell=fm.n_cell, n_gran=fm.n_gran,
+        cell_center=np.ascontiguousarray(fm.cell_center[cp]),
+        cell_volume=np.ascontiguousarray(fm.cell_volume[cp]),
+        cell_gran=remap(fm.cell_gran[cp], finv),
+        gran_normal=np.ascontiguousarray(fm.gran_normal[gp]),
+        gran_square=np.ascontiguousarray(fm.gran_square[gp]),
+        gran_center=np.ascontiguousarray(fm.gran_center[gp]),
+        gran_neighbour=remap(fm.gran_neighbour[gp], cinv),
+        gran_neighbour_tvd=remap(fm.gran_neighbour_tvd[gp], cinv),
+        gran_type=np.ascontiguousarray(fm.gran_type[gp]).astype(np.int32))
+
+
 def _axis_block(A, n, cell_at, axis, centre, area, boundary_type):
     """Faces along one axis.  A = face position 0..n along the axis."""
     m = len(A)

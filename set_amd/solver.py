@@ -28,7 +28,7 @@ PD = ctypes.POINTER(D)
 PI = ctypes.POINTER(I32)
 
 OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_RHO, ERR_REGION, ERR_FACE_RHO, ERR_UNSUPPORTED = range(8)
-F_ZERO_BW, F_AXISYM, F_SOURCE_MK, F_HELIO_HACKS, F_FAST = 1, 2, 4, 8, 16
+F_ZERO_BW, F_AXISYM, F_SOURCE_MK, F_HELIO_HACKS, F_FAST, F_REORDER = 1, 2, 4, 8, 16, 32
 RIEMANN_LAX, RIEMANN_HLL, RIEMANN_HLLC, RIEMANN_HLLD = range(4)
 TIME_EULER, TIME_HEUN = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
@@ -42,6 +42,7 @@ EXPORTS = (
     "setmhd_batch_add", "setmhd_batch_remove", "setmhd_batch_step", "setmhd_batch_step_async",
     "setmhd_batch_poll", "setmhd_batch_wait", "setmhd_batch_query",
     "setmhd_batch_download", "setmhd_batch_autotune", "setmhd_batch_set_launch", "setmhd_mem_info",
+    "setmhd_mesh_order",
 )
 
 
@@ -107,6 +108,7 @@ def load():
     L.setmhd_batch_autotune.argtypes = [P, I32, ctypes.POINTER(I32), ctypes.POINTER(I32),
                                         ctypes.POINTER(I32)]
     L.setmhd_mem_info.argtypes = [P, I64P, I64P]
+    L.setmhd_mesh_order.argtypes = [I32, I32, PD, PI, PI, PI]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is not None and name not in ("setmhd_config_reference",
@@ -134,6 +136,20 @@ def cartesian_config(cfl=0.4, r_hold=0.1, precision=PREC_FP64, time_scheme=TIME_
     c.device = device
     c.kchunk = kchunk
     return c
+
+
+def mesh_order(fm):
+    """The device numbering SETMHD_F_REORDER gives face-list mesh fm
+    (setmhd_mesh_order, host only): (cell_perm, gran_perm), 0-based, entry
+    i being the caller's cell or face placed i-th."""
+    cc = np.ascontiguousarray(fm.cell_center, np.float64)
+    nb = np.ascontiguousarray(fm.gran_neighbour, np.int32)
+    cp = np.zeros(fm.n_cell, np.int32)
+    gp = np.zeros(fm.n_gran, np.int32)
+    st = load().setmhd_mesh_order(fm.n_cell, fm.n_gran, _pd(cc), _pi(nb), _pi(cp), _pi(gp))
+    if st:
+        raise SetMHDError(st, "mesh_order: first face neighbour out of range or empty mesh")
+    return cp, gp
 
 
 def _pd(a):

@@ -152,6 +152,56 @@ def test_fast_build_hlld_plain_2d(nsteps, bound, order):
     assert t == pytest.approx(s.t, rel=1e-12)
 
 
+@pytest.mark.parametrize("which", ["reference", "cartesian3d"])
+def test_reorder_bitwise(which):
+    """SETMHD_F_REORDER on a randomly renumbered mesh: the device holds its
+    own (Morton) numbering while every call takes and returns the caller's.
+    State, snapshot, face fluxes, probes, time and next dt equal bit for bit
+    those of the original mesh run without reordering, mapped through the
+    renumbering; so does the renumbered mesh run without reordering."""
+    if which == "reference":
+        fm, W = helio_mesh(120, 60)
+        ocfg, nsteps, dt0 = oracle.reference_config(), 12, 2.0e-9
+    else:
+        g = mesh.grid_3d(24, 20, 16)
+        fm = mesh.face_mesh(g)
+        W = mesh.heliosphere_state(g, wind=mesh.Wind(r_in=0.3))
+        ocfg, nsteps, dt0 = oracle.cartesian_config(), 10, None
+    case(f"dropin reorder {which} {fm.n_cell} cells, {nsteps} steps")
+    rng = np.random.default_rng(4)
+    cp, gp = rng.permutation(fm.n_cell), rng.permutation(fm.n_gran)
+    fr, Wr = mesh.renumber_face_mesh(fm, cp, gp), W[cp]
+    probe = np.array([1, fm.n_cell // 3, fm.n_cell])  # numbers in the renumbered mesh
+    out = {}
+    for name, m, w, flags, ids in (("base", fm, W, 0, cp[probe - 1] + 1),
+                                   ("renumbered", fr, Wr, 0, probe),
+                                   ("reordered", fr, Wr, solver.F_REORDER, probe)):
+        cfg = gpu_cfg(ocfg)
+        cfg.flags |= flags
+        with solver.Context(cfg) as c:
+            c.upload_mesh(m, w)
+            if dt0 is None:
+                dt0 = c.dt_from_state()
+            c.set_dt(dt0)
+            c.step(nsteps)
+            c.download_async()
+            cpar, t = c.download()
+            snap, ts = c.download_wait()
+            _, dtn = c.sync()
+            out[name] = dict(cp=cpar, snap=snap, t=t, ts=ts, dtn=dtn, pot=c.download_potok(),
+                             probe=c.probe(ids)[0])
+    b, r, o = out["base"], out["renumbered"], out["reordered"]
+    np.testing.assert_array_equal(r["cp"], b["cp"][cp])
+    np.testing.assert_array_equal(r["pot"], b["pot"][gp])
+    for k in ("cp", "snap", "pot", "probe"):
+        np.testing.assert_array_equal(o[k], r[k])
+    np.testing.assert_array_equal(o["snap"], o["cp"])
+    np.testing.assert_array_equal(o["probe"], b["probe"])
+    np.testing.assert_array_equal(o["probe"], o["cp"][probe - 1, :8])
+    assert o["t"] == r["t"] == b["t"] == o["ts"]
+    assert o["dtn"] == r["dtn"] == b["dtn"]
+
+
 def test_density_failure_reported():
     case("dropin forced rho<0 -> SETMHD_ERR_RHO")
     g = mesh.grid_2d(40, 20)
