@@ -18,8 +18,12 @@
 ! setmhd_download_async and written at the next print, so its copy overlaps
 ! the steps in between.
 !
+! With 'reorder' the device holds the mesh renumbered for locality
+! (SETMHD_F_REORDER); files, probes and dumps keep the input's numbering and
+! the results are the same bits.
+!
 ! usage: fcmhd_driver input.bin output.bin all_step [print_step] [restart.bin|-]
-!                     [cartesian|reference] [probes.txt|-] [dump]
+!                     [cartesian|reference] [probes.txt|-] [dump|-] [reorder]
 program fcmhd_driver
   use iso_c_binding
   use setmhd
@@ -38,7 +42,7 @@ program fcmhd_driver
   integer(4), allocatable :: probes(:)
   real(8), allocatable :: pv(:, :)
   logical :: dump
-  logical :: cartesian
+  logical :: cartesian, reorder
   type(setmhd_config) :: cfg
   type(c_ptr) :: ctx
   integer(c_int) :: st
@@ -46,7 +50,7 @@ program fcmhd_driver
   nargs = command_argument_count()
   if (nargs < 3) then
     print *, 'usage: fcmhd_driver input.bin output.bin all_step [print_step] [restart.bin|-] ', &
-             '[cartesian|reference] [probes.txt|-] [dump]'
+             '[cartesian|reference] [probes.txt|-] [dump|-] [reorder]'
     stop 2
   end if
   call get_command_argument(1, fin)
@@ -73,6 +77,11 @@ program fcmhd_driver
   if (nargs >= 8) then
     call get_command_argument(8, arg)
     dump = trim(arg) == 'dump'
+  end if
+  reorder = .false.
+  if (nargs >= 9) then
+    call get_command_argument(9, arg)
+    reorder = trim(arg) == 'reorder'
   end if
   nprobe = 0
   if (len_trim(fprobes) > 0) then
@@ -138,6 +147,7 @@ program fcmhd_driver
   else
     call setmhd_config_reference(cfg)
   end if
+  if (reorder) cfg%flags = ior(cfg%flags, SETMHD_F_REORDER)
   st = setmhd_create(ctx, cfg)
   if (st /= SETMHD_OK) stop 'setmhd_create failed'
   st = setmhd_upload_mesh(ctx, host_N_cell, host_N_gran, host_Cell_par, host_Cell_center, &

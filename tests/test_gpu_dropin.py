@@ -302,3 +302,14 @@ def test_fortran_driver_outputs(tmp_path):
         a = [l.rstrip() for l in (tmp_path / f_out).read_text().splitlines()]
         b = [l.rstrip() for l in (tmp_path / py).read_text().splitlines()]
         assert a == b, (f_out, next((x, y) for x, y in zip(a, b) if x != y) if len(a) == len(b) else len(a))
+    # the same run with the mesh renumbered on the device: every output file
+    # (state, probe series, dumps) byte for byte the same
+    rdir = tmp_path / "reorder"
+    rdir.mkdir()
+    (rdir / "probes.txt").write_text((tmp_path / "probes.txt").read_text())
+    r = subprocess.run([exe, str(fin), str(rdir / "out.bin"), "12", "5", "-", "reference", "probes.txt", "dump",
+                        "reorder"], capture_output=True, text=True, timeout=120, cwd=rdir)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    for name in ("out.bin", "1D_rhoT.txt", "output_data_3.txt", "output_data.txt"):
+        assert (rdir / name).read_bytes() == (tmp_path / name).read_bytes(), name
