@@ -55,7 +55,7 @@ enum setmhd_status {
 #define SETMHD_F_FAST 16       /* drop-in kernels with FMA contraction and reciprocal-based
                                   division: same algorithm, agrees to rounding, not bit for bit */
 #define SETMHD_F_REORDER 32    /* face-list meshes: the device holds the cells in Morton order
-                                  of their centres and the faces by their first cell
+                                  of their centres and the faces by normal axis and first cell
                                   (setmhd_mesh_order); every call still takes and returns the
                                   caller's numbering.  Results are bit for bit the same. */
 
@@ -128,11 +128,14 @@ int setmhd_upload_mesh(setmhd_ctx *ctx, int32_t n_cell, int32_t n_gran,
  * callers that renumber their own files: cell_perm[i] (0-based) is the
  * caller's cell placed i-th, cells sorted by the Morton (Z-order) key of
  * their centre in the mesh's bounding box, 21 bits per axis, ties by
- * number; gran_perm[f] is the caller's face placed f-th, faces sorted by
- * the new place of their first cell (gran_neighbour(1,:)), ties by number.
- * Host only (no device call).  No counterpart in the reference, whose
- * kernels take the mesh in the order Setka.cpp:1158-1567 writes it. */
+ * number; gran_perm[f] is the caller's face placed f-th, faces grouped by
+ * the axis their normal is closest to (x, y, z; ties to the lower axis),
+ * then sorted by the new place of their first cell (gran_neighbour(1,:)),
+ * ties by number.  Host only (no device call).  No counterpart in the
+ * reference, whose kernels take the mesh in the order Setka.cpp:1158-1567
+ * writes it. */
 int setmhd_mesh_order(int32_t n_cell, int32_t n_gran, const double *cell_center /*[3*n_cell]*/,
+                      const double *gran_normal /*[3*n_gran]*/,
                       const int32_t *gran_neighbour /*[2*n_gran]*/,
                       int32_t *cell_perm /*[n_cell]*/, int32_t *gran_perm /*[n_gran]*/);
 

@@ -58,11 +58,11 @@ module setmhd
       real(c_double), value :: t0
     end function
 
-    integer(c_int) function setmhd_mesh_order(n_cell, n_gran, cell_center, gran_neighbour, &
-        cell_perm, gran_perm) bind(C, name="setmhd_mesh_order")
+    integer(c_int) function setmhd_mesh_order(n_cell, n_gran, cell_center, gran_normal, &
+        gran_neighbour, cell_perm, gran_perm) bind(C, name="setmhd_mesh_order")
       import :: c_int, c_int32_t, c_double
       integer(c_int32_t), value :: n_cell, n_gran
-      real(c_double), intent(in) :: cell_center(*)
+      real(c_double), intent(in) :: cell_center(*), gran_normal(*)
       integer(c_int32_t), intent(in) :: gran_neighbour(*)
       integer(c_int32_t), intent(out) :: cell_perm(*), gran_perm(*)
     end function

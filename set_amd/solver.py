@@ -108,7 +108,7 @@ def load():
     L.setmhd_batch_autotune.argtypes = [P, I32, ctypes.POINTER(I32), ctypes.POINTER(I32),
                                         ctypes.POINTER(I32)]
     L.setmhd_mem_info.argtypes = [P, I64P, I64P]
-    L.setmhd_mesh_order.argtypes = [I32, I32, PD, PI, PI, PI]
+    L.setmhd_mesh_order.argtypes = [I32, I32, PD, PD, PI, PI, PI]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is not None and name not in ("setmhd_config_reference",
@@ -143,10 +143,11 @@ def mesh_order(fm):
     (setmhd_mesh_order, host only): (cell_perm, gran_perm), 0-based, entry
     i being the caller's cell or face placed i-th."""
     cc = np.ascontiguousarray(fm.cell_center, np.float64)
+    gn = np.ascontiguousarray(fm.gran_normal, np.float64)
     nb = np.ascontiguousarray(fm.gran_neighbour, np.int32)
     cp = np.zeros(fm.n_cell, np.int32)
     gp = np.zeros(fm.n_gran, np.int32)
-    st = load().setmhd_mesh_order(fm.n_cell, fm.n_gran, _pd(cc), _pi(nb), _pi(cp), _pi(gp))
+    st = load().setmhd_mesh_order(fm.n_cell, fm.n_gran, _pd(cc), _pd(gn), _pi(nb), _pi(cp), _pi(gp))
     if st:
         raise SetMHDError(st, "mesh_order: first face neighbour out of range or empty mesh")
     return cp, gp

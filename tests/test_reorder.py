@@ -5,7 +5,8 @@ helper the GPU tests use, pinned on the CPU.
   * the order is the Z-order curve: on a 4x4 lattice it visits the cells in
     the textbook Morton sequence, and on scattered 3-D points it sorts by a
     bit-by-bit interleave written out here independently;
-  * faces follow the new place of their first cell;
+  * faces are grouped by the axis their normal is closest to, then follow
+    the new place of their first cell;
   * the oracle on a randomly renumbered mesh gives the same bits, mapped
     back, as on the original mesh: the step does not depend on the
     numbering (reference semantics with TVD, Heun and sources, and 3-D
@@ -24,6 +25,9 @@ def test_morton_order_on_a_lattice():
     # (i, j) -> i + 4 j visited in Z order: x is the lowest interleaved bit
     assert cp.tolist() == [0, 1, 4, 5, 2, 3, 6, 7, 8, 9, 12, 13, 10, 11, 14, 15]
     assert sorted(gp.tolist()) == list(range(fm.n_gran))
+    # the 20 x-faces (normals +-x) come first, then the 20 y-faces
+    assert np.all(np.abs(fm.gran_normal[gp[:20], 0]) == 1.0)
+    assert np.all(np.abs(fm.gran_normal[gp[20:], 1]) == 1.0)
 
 
 def _interleave_key(q):
@@ -43,8 +47,11 @@ def test_order_matches_bitwise_interleave():
     centre = rng.uniform(-3.0, 7.0, (n, 3))
     centre[:7] = centre[7]  # ties keep the caller's order
     nb = np.stack([rng.integers(1, n + 1, 900), np.full(900, -1)], 1).astype(np.int32)
-    fake = mesh.FaceMesh(n, 900, centre, np.ones(n), np.zeros((n, 6), np.int32), fm.gran_normal[:1],
-                         np.ones(1), np.zeros((1, 3)), nb, nb, np.ones(900, np.int32))
+    normal = rng.normal(size=(900, 3))
+    normal[:30] = [0.6, -0.6, 0.1]  # a tie between x and y goes to x
+    normal /= np.linalg.norm(normal, axis=1)[:, None]
+    fake = mesh.FaceMesh(n, 900, centre, np.ones(n), np.zeros((n, 6), np.int32), normal,
+                         np.ones(900), np.zeros((900, 3)), nb, nb, np.ones(900, np.int32))
     cp, gp = solver.mesh_order(fake)
     lo, hi = centre.min(0), centre.max(0)
     q = np.clip((centre - lo) / (hi - lo), 0.0, 1.0) * float((1 << 21) - 1)
@@ -54,7 +61,8 @@ def test_order_matches_bitwise_interleave():
     cinv = np.empty(n, np.int64)
     cinv[cp] = np.arange(n)
     first = cinv[nb[:, 0] - 1]
-    assert gp.tolist() == np.lexsort((np.arange(900), first)).tolist()
+    axis = np.argmax(np.abs(normal), axis=1)  # first maximum: ties to the lower axis
+    assert gp.tolist() == np.lexsort((np.arange(900), first, axis)).tolist()
 
 
 def test_order_rejects_bad_neighbours():
