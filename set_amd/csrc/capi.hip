@@ -385,13 +385,19 @@ int grid_kchunk(const setmhd_ctx* c, int nx, int ny, int nz, bool batch) {
   return (int)std::min(32LL, std::max(8LL, kc));
 }
 
-// Default z-march tiles of a grid alone: 16x16 in FP64, 64x4 in the mixed
-// form (256^3: +1.5 % and +2 % against 32x8, the mixed form's plane-DMA shape)
+// Default z-march tiles of a grid alone: 16x16 in FP64 (256^3: +1.5 %
+// against 32x8); in the mixed form 32x8, whose planes arrive by LDS DMA,
+// where rows have even length (256^3: +3 % against 64x4), else 64x4
+bool mixed_dma_tile(const setmhd_ctx* c) { return c->sg.nx % 2 == 0; }
 int tile_x(const setmhd_ctx* c) {
-  return c->cfg.tile_x > 0 ? c->cfg.tile_x : (c->cfg.precision == SETMHD_PREC_FP64 ? 16 : 64);
+  if (c->cfg.tile_x > 0) return c->cfg.tile_x;
+  if (c->cfg.precision == SETMHD_PREC_FP64) return 16;
+  return mixed_dma_tile(c) ? 32 : 64;
 }
 int tile_y(const setmhd_ctx* c) {
-  return c->cfg.tile_y > 0 ? c->cfg.tile_y : (c->cfg.precision == SETMHD_PREC_FP64 ? 16 : 4);
+  if (c->cfg.tile_y > 0) return c->cfg.tile_y;
+  if (c->cfg.precision == SETMHD_PREC_FP64) return 16;
+  return mixed_dma_tile(c) ? 8 : 4;
 }
 
 // drop-in kernels: the exact build, or the fast one under SETMHD_F_FAST

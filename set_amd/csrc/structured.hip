@@ -145,14 +145,14 @@ struct PickLds {
 // face (the face on the low side of cell (ry, rx)).
 template <int D, class R, int TX, int TY>
 __device__ __forceinline__ void lds_face(Smem<R, TX, TY>& sm, int ly, int lx, int ry, int rx,
-                                         double& smax, bool& ok) {
+                                         R& smax, bool& ok) {
   constexpr int D1 = D + 1, D2 = D == 0 ? 2 : 0;
   const int offL = ly * (TX + 2) + lx, offR = ry * (TX + 2) + rx;
   const Side<R> L = get_side<D>(sm, offL);
   const Side<R> Rr = get_side<D>(sm, offR);
   R F[8], s;
   ok &= hlld(L, Rr, F, s, PickLds<D, R, TX, TY>{sm, offL, offR});
-  smax = fmax(smax, (double)s);
+  smax = fmax(smax, s);
   auto& dst = [&]() -> auto& {
     if constexpr (D == 0) return sm.fx;
     else return sm.fy;
@@ -330,7 +330,7 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
     hgj = min(max(hgj, 0), g.ny - 1);
   }
 
-  double smax[3] = {0.0, 0.0, 0.0};
+  R smax[3] = {R(0), R(0), R(0)};  // wave speeds in R: max commutes with the exact widening
   double rs_d = 0.0, rs_r = 0.0;
   double accp[8];
   Side<R> zp;
@@ -435,7 +435,7 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
       {  // z-face below this cell
         R s;
         ok &= hlld(zp, zc, Fz, s);
-        smax[2] = fmax(smax[2], (double)s);
+        smax[2] = fmax(smax[2], s);
       }
       // z-face: rotate frame (z, x, y) back to x, y, z
       double fz[8];
@@ -480,9 +480,9 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
   const bool want_rs = a.resid != nullptr && MODE != kModeHeun1;
   double cand = kDtReset;
   if (MODE != kModeHeun2) {
-    cand = g.cfl * g.distx / smax[0];
-    cand = fmin(cand, g.cfl * g.disty / smax[1]);
-    cand = fmin(cand, g.cfl * g.distz / smax[2]);
+    cand = g.cfl * g.distx / (double)smax[0];
+    cand = fmin(cand, g.cfl * g.disty / (double)smax[1]);
+    cand = fmin(cand, g.cfl * g.distz / (double)smax[2]);
     cand = fmin(cand, g.dt_cap);
     cand = wave_min(cand);
   }
@@ -574,7 +574,7 @@ __device__ __forceinline__ TileRed row_tile(const FusedArgs& a, int b, bool book
     a.dt.slot[(a.cur + 2) % 3] = kDtReset;      // :790, for step n+2
   }
   const double xc = __dadd_rn(g.x0, __dmul_rn((double)cc + 0.5, g.dx));
-  double smax[2] = {0.0, 0.0};
+  R smax[2] = {R(0), R(0)};
   double rs_d = 0.0, rs_r = 0.0;
   double accp[8];
   bool ok = true;
@@ -614,7 +614,7 @@ __device__ __forceinline__ TileRed row_tile(const FusedArgs& a, int b, bool book
         if (l < RW - 1) {  // x-face on the right of this lane's cell
           R F[8], s;
           ok &= hlld(side_of(qc, 0), row_side(sm, l + 1), F, s);
-          smax[0] = fmax(smax[0], (double)s);
+          smax[0] = fmax(smax[0], s);
 #pragma unroll
           for (int v = 0; v < 8; v++) sm.fx[v][l] = F[v];
         }
@@ -623,7 +623,7 @@ __device__ __forceinline__ TileRed row_tile(const FusedArgs& a, int b, bool book
       {  // y-face below this lane's cell
         R s;
         ok &= hlld(yp, yc, Fy, s);
-        smax[1] = fmax(smax[1], (double)s);
+        smax[1] = fmax(smax[1], s);
       }
       double fy[8];
       from_y_frame(Fy, fy);
@@ -654,8 +654,8 @@ __device__ __forceinline__ TileRed row_tile(const FusedArgs& a, int b, bool book
   const bool want_rs = a.resid != nullptr && MODE != kModeHeun1;
   double cand = kDtReset;
   if (MODE != kModeHeun2) {
-    cand = g.cfl * g.distx / smax[0];
-    cand = fmin(cand, g.cfl * g.disty / smax[1]);
+    cand = g.cfl * g.distx / (double)smax[0];
+    cand = fmin(cand, g.cfl * g.disty / (double)smax[1]);
     cand = fmin(cand, g.dt_cap);
     cand = wave_min(cand);
   }
@@ -740,7 +740,7 @@ __device__ __forceinline__ TileRed wrow_march(const FusedArgs& a, int t, bool ac
     a.dt.slot[(a.cur + 2) % 3] = kDtReset;      // :790, for step n+2
   }
   const double xc = __dadd_rn(g.x0, __dmul_rn((double)cc + 0.5, g.dx));
-  double smax[2] = {0.0, 0.0};
+  R smax[2] = {R(0), R(0)};
   double rs_d = 0.0, rs_r = 0.0;
   double accp[8];
   bool ok = true;
@@ -767,14 +767,14 @@ __device__ __forceinline__ TileRed wrow_march(const FusedArgs& a, int t, bool ac
         const bool okx = hlld(xs, xr, Fx, s);
         if (lane < 63) {
           ok &= okx;
-          smax[0] = fmax(smax[0], (double)s);
+          smax[0] = fmax(smax[0], s);
         }
       }
       R Fy[8];
       {  // y-face below this lane's cell
         R s;
         ok &= hlld(yp, yc, Fy, s);
-        smax[1] = fmax(smax[1], (double)s);
+        smax[1] = fmax(smax[1], s);
       }
       double fy[8];
       from_y_frame(Fy, fy);
@@ -805,8 +805,8 @@ __device__ __forceinline__ TileRed wrow_march(const FusedArgs& a, int t, bool ac
   const bool want_rs = a.resid != nullptr && MODE != kModeHeun1;
   double cand = kDtReset;
   if (MODE != kModeHeun2 && active) {
-    cand = g.cfl * g.distx / smax[0];
-    cand = fmin(cand, g.cfl * g.disty / smax[1]);
+    cand = g.cfl * g.distx / (double)smax[0];
+    cand = fmin(cand, g.cfl * g.disty / (double)smax[1]);
     cand = fmin(cand, g.dt_cap);
   }
   cand = wave_min(cand);
