@@ -69,6 +69,9 @@ def lib():
                                     PD, PD, PD, PD, ctypes.c_int]
         L.orc_cell_pass.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(Mesh), ctypes.c_int,
                                     D, PD, PD, PD, ctypes.c_int]
+        L.orc_face_states.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(Mesh), ctypes.c_int,
+                                      ctypes.c_int, PD, PD, PD, PD]
+        L.orc_face_states.restype = ctypes.c_int
         L.orc_dt_scan.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(Mesh), PD, ctypes.c_int]
         L.orc_dt_scan.restype = D
         L.orc_run.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(Mesh), PD, PD, PD, PD, PD,
@@ -154,6 +157,14 @@ class Solver:
                                  _pd(self.W), _pd(self.W2), _pd(self.potok), ctypes.byref(d),
                                  self.nthreads)
         return st, d.value
+
+    def face_states(self, f, stage=0):
+        """(status, q1, q2): the two primitive states face f (0-based) hands
+        to chlld in the given stage."""
+        q1, q2 = np.zeros(8), np.zeros(8)
+        st = lib().orc_face_states(ctypes.byref(self.cfg), ctypes.byref(self.mesh), int(f), int(stage),
+                                   _pd(self.W), _pd(self.W2), _pd(q1), _pd(q2))
+        return st, q1, q2
 
     def cell_pass(self, stage, dt):
         return lib().orc_cell_pass(ctypes.byref(self.cfg), ctypes.byref(self.mesh), int(stage),

@@ -68,41 +68,30 @@ void orc_source_mk(const double *cp, double *out) {
   out[2] = sum3;
 }
 
-/* One face of CUF_MGD_grans_RK2 (cuf_solvers.cuf:243-440); the Euler
- * kernel CUF_MGD_grans (:62-241) is the same with stage 0 reading W.
- * Returns 1 if the face was skipped (inner-sphere test), 0 if computed,
- * <0 on a reference STOP condition. */
-static int face_one(const orc_config *cfg, const orc_mesh *m, int gr,
-                    int stage, const double *W, const double *W2,
-                    double *potok, double *time_out) {
+/* The two states of face gr that CUF_MGD_grans_RK2 (cuf_solvers.cuf:
+ * 300-420) hands to chlld: the cells' primitives, the boundary ghost of
+ * types 4/5/wall/1/3 (:318-354) and the minmod TVD reconstruction with its
+ * positivity fallback (:363-412).  Returns 1 if the face is skipped (the
+ * inner-sphere test, :282-284), 0, or ORC_ERR_FACE_RHO (:414-420). */
+int orc_face_states(const orc_config *cfg, const orc_mesh *m, int gr, int stage,
+                    const double *W, const double *W2, double *q1, double *q2) {
   const double *S = (stage == 2) ? W2 : W;
-  double time = cfg->dt_cap; /* 'time = 100000.0' */
   int type = m->gran_type[gr];
   int s1 = m->gran_neighbour[2 * gr + 0];
   int s2 = m->gran_neighbour[2 * gr + 1];
   const double *c1 = &m->cell_center[3 * (s1 - 1)];
   if (norm2v(c1) < cfg->r_skip_face) return 1; /* :282-284 */
-
   const double *gc = &m->gran_center[3 * gr];
-  double distant[3] = {gc[0] - c1[0], gc[1] - c1[1], gc[2] - c1[2]};
-  double dist = norm2v(distant); /* :291-293 */
   const double *nrm = &m->gran_normal[3 * gr];
-
-  int metod = cfg->riemann;
-  if ((cfg->flags & ORC_F_HELIO_HACKS) && gc[0] > F4(0.22) && gc[1] < F4(0.1) &&
-      fabs(nrm[0]) < F4(0.1))
-    metod = 0; /* :296-298 */
-
-  double q1[8], q2[8];
   memcpy(q1, &S[NPAR * (s1 - 1)], 8 * sizeof(double)); /* :305-309 */
   const double *c2 = c1;
   int tvd = 1;
   if (type == 4) { /* :318-324 */
-    memcpy(q2, q1, sizeof q1);
+    memcpy(q2, q1, 8 * sizeof(double));
     if ((cfg->flags & ORC_F_HELIO_HACKS) && q2[1] > F4(-0.1)) q2[1] = F4(-0.1);
     tvd = 0;
   } else if (type == 5) { /* :325-329 */
-    memcpy(q2, q1, sizeof q1);
+    memcpy(q2, q1, 8 * sizeof(double));
     q2[2] = -q2[2];
     tvd = 0;
   } else if (type == 2 && s2 == -1) { /* :330-345 */
@@ -112,20 +101,18 @@ static int face_one(const orc_config *cfg, const orc_mesh *m, int gr,
       vel1[k] = q1[1 + k] - vel * nrm[k];
       vel2[k] = vel * nrm[k];
     }
-    memcpy(q2, q1, sizeof q1);
+    memcpy(q2, q1, 8 * sizeof(double));
     for (int k = 0; k < 3; k++) q2[1 + k] = vel1[k] - vel2[k];
     tvd = 0;
   } else if (s2 < 1) {
     /* Types 1 and 3 with s2 = -1 reach Cell_par(:, s2) unguarded in the
      * reference (:346-354); stated here as an outflow copy ghost. */
-    memcpy(q2, q1, sizeof q1);
+    memcpy(q2, q1, 8 * sizeof(double));
     tvd = 0;
   } else { /* :346-354 */
     memcpy(q2, &S[NPAR * (s2 - 1)], 8 * sizeof(double));
     c2 = &m->cell_center[3 * (s2 - 1)];
   }
-  double square = m->gran_square[gr];
-
   int ss1 = m->gran_neighbour_tvd[2 * gr + 0];
   int ss2 = m->gran_neighbour_tvd[2 * gr + 1];
   if (cfg->recon == 1 && tvd && ss1 >= 1 && ss2 >= 1) { /* :363-412 */
@@ -147,8 +134,8 @@ static int face_one(const orc_config *cfg, const orc_mesh *m, int gr,
     const double *q11 = &S[NPAR * (ss1 - 1)];
     const double *q22 = &S[NPAR * (ss2 - 1)];
     double t1[8], t2[8];
-    memcpy(t1, q1, sizeof q1);
-    memcpy(t2, q2, sizeof q2);
+    memcpy(t1, q1, sizeof t1);
+    memcpy(t2, q2, sizeof t2);
     for (int i = 0; i < 5; i++) {
       t1[i] = orc_linear(-dff1, q11[i], -df1, q1[i], df2, q2[i], 0.0);
       t2[i] = orc_linear(-dff2, q22[i], -df2, q2[i], df1, q1[i], 0.0);
@@ -157,10 +144,35 @@ static int face_one(const orc_config *cfg, const orc_mesh *m, int gr,
     if (t1[4] < 0.0) t1[4] = q1[4];
     if (t2[0] < 0.0) t2[0] = q2[0];
     if (t2[4] < 0.0) t2[4] = q2[4];
-    memcpy(q1, t1, sizeof q1);
-    memcpy(q2, t2, sizeof q2);
+    memcpy(q1, t1, sizeof t1);
+    memcpy(q2, t2, sizeof t2);
   }
   if (q1[0] < 0.0 || q2[0] < 0.0) return ORC_ERR_FACE_RHO; /* :414-420 */
+  return 0;
+}
+
+/* One face of CUF_MGD_grans_RK2 (cuf_solvers.cuf:243-440); the Euler
+ * kernel CUF_MGD_grans (:62-241) is the same with stage 0 reading W.
+ * Returns 1 if the face was skipped (inner-sphere test), 0 if computed,
+ * <0 on a reference STOP condition. */
+static int face_one(const orc_config *cfg, const orc_mesh *m, int gr,
+                    int stage, const double *W, const double *W2,
+                    double *potok, double *time_out) {
+  double time = cfg->dt_cap; /* 'time = 100000.0' */
+  double q1[8], q2[8];
+  int st0 = orc_face_states(cfg, m, gr, stage, W, W2, q1, q2);
+  if (st0 != 0) return st0;
+  const double *c1 = &m->cell_center[3 * (m->gran_neighbour[2 * gr] - 1)];
+  const double *gc = &m->gran_center[3 * gr];
+  double distant[3] = {gc[0] - c1[0], gc[1] - c1[1], gc[2] - c1[2]};
+  double dist = norm2v(distant); /* :291-293 */
+  const double *nrm = &m->gran_normal[3 * gr];
+  double square = m->gran_square[gr];
+
+  int metod = cfg->riemann;
+  if ((cfg->flags & ORC_F_HELIO_HACKS) && gc[0] > F4(0.22) && gc[1] < F4(0.1) &&
+      fabs(nrm[0]) < F4(0.1))
+    metod = 0; /* :296-298 */
 
   double wc = 0.0, dsl, dsp, dsc, POTOK[8];
   int st = orc_chlld(metod, nrm[0], nrm[1], nrm[2], wc, q1, q2, &dsl, &dsp,
@@ -231,7 +243,11 @@ static int cell_one(const orc_config *cfg, const orc_mesh *m, int gr,
   if (smk) orc_source_mk(qqq, source); /* :684 */
 
   double POTOK[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int i = 0; i < 6; i++) { /* :691-703 (4 slots in 2-D, 6 in 3-D) */
+  /* :691-703 sums slots 1..4 ('do i = 1, 4', also :491); SET's meshes hold
+   * 0 in slots 5..6 (Setka.cpp:1344-1396), where summing all six is the same.
+   * The six-slot sum is this build's extension for 3-D face-list meshes,
+   * whose z- and z+ faces sit in slots 5 and 6. */
+  for (int i = 0; i < 6; i++) {
     int j = m->cell_gran[6 * gr + i];
     if (j == 0) continue;
     const double *f = &potok[9 * (j - 1)];

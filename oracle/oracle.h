@@ -83,6 +83,13 @@ double orc_linear(double x1, double t1, double x2, double t2, double x3,
                   double t3, double y);
 void orc_source_mk(const double *cell_par16, double *out3);
 
+/* The two states face gr (0-based) hands to chlld in stage `stage`:
+ * boundary ghosts and the minmod TVD reconstruction with its positivity
+ * fallback (cuf_solvers.cuf:300-420).  Returns 1 if the face is skipped,
+ * 0, or ORC_ERR_FACE_RHO.                                                */
+int orc_face_states(const orc_config *cfg, const orc_mesh *m, int gr, int stage,
+                    const double *W, const double *W2, double *q1, double *q2);
+
 /* passes over the face list / cell list.
  * stage: 0 = Euler kernels, 1/2 = Heun stage.  W = Cell_par (16*Nc),
  * W2 = Cell_par2 (16*Nc).  potok = Gran_POTOK (9*Nf).  dt_min is the

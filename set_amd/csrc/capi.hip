@@ -35,7 +35,8 @@ struct setmhd_ctx {
   int kind = 0;  // 0 none, 1 unstructured, 2 structured
   long long n_cell = 0, n_gran = 0;
   long long step = 0;  // steps done since upload (selects the dt slot)
-  // device scalars: dt ring [3], time [1], error word, residual [2]
+  // device scalars: dt ring [3], time [1], residual [2]; error words: [0] the
+  // steps', [1] the batch admissions' (their scans run beside a step)
   double* d_scal = nullptr;
   int* d_err = nullptr;
   // unstructured
@@ -624,7 +625,8 @@ int setmhd_create(setmhd_ctx** out, const setmhd_config* cfg) {
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
     return bad(SETMHD_ERR_HIP);
   if (hipMalloc((void**)&c->d_scal, 8 * sizeof(double)) != hipSuccess) return bad(SETMHD_ERR_NOMEM);
-  if (hipMalloc((void**)&c->d_err, sizeof(int)) != hipSuccess) return bad(SETMHD_ERR_NOMEM);
+  if (hipMalloc((void**)&c->d_err, 2 * sizeof(int)) != hipSuccess) return bad(SETMHD_ERR_NOMEM);
+  if (hipMemset(c->d_err, 0, 2 * sizeof(int)) != hipSuccess) return bad(SETMHD_ERR_HIP);
   if (upload_scalars(c, 0.0) != SETMHD_OK) return bad(SETMHD_ERR_HIP);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
@@ -682,7 +684,9 @@ int setmhd_upload_mesh(setmhd_ctx* c, int32_t n_cell, int32_t n_gran, const doub
   if (c->cfg.precision != SETMHD_PREC_FP64)
     return fail(c, SETMHD_ERR_UNSUPPORTED, "the face-list path is FP64 only");
   // The validation of Set_Storage (Storage_module.f90:92-156): neighbour 1
-  // in 1..N_cell, neighbour 2 and TVD neighbours <= N_cell, density >= 1e-9.
+  // in 1..N_cell, neighbour 2 and TVD neighbours <= N_cell, the neutral
+  // moments Cell_par(9), (12), (13), (16) >= 0 and density >= 1e-9, compared
+  // as the reference compares them (its 1e-9 is a default-REAL literal).
   for (int32_t f = 0; f < n_gran; f++) {
     int32_t s1 = gran_neighbour[2 * f], s2 = gran_neighbour[2 * f + 1];
     if (s1 < 1 || s1 > n_cell || s2 > n_cell || s2 == 0 || s2 < -1 ||
@@ -690,7 +694,10 @@ int setmhd_upload_mesh(setmhd_ctx* c, int32_t n_cell, int32_t n_gran, const doub
       return fail(c, SETMHD_ERR_ARG, "upload_mesh: face neighbour out of range");
   }
   for (int32_t i = 0; i < n_cell; i++) {
-    if (cell_par[16 * (size_t)i] < 0.000000001)
+    const double* cp = cell_par + 16 * (size_t)i;
+    if (cp[8] < 0.0 || cp[11] < 0.0 || cp[12] < 0.0 || cp[15] < 0.0)
+      return fail(c, SETMHD_ERR_ARG, "upload_mesh: Cell_par(9/12/13/16) < 0");
+    if (cp[0] < SETMHD_F4(0.000000001))
       return fail(c, SETMHD_ERR_ARG, "upload_mesh: rho < 1e-9");
     for (int s = 0; s < 6; s++)
       if (cell_gran[6 * (size_t)i + s] < 0 || cell_gran[6 * (size_t)i + s] > n_gran)
@@ -882,6 +889,9 @@ int setmhd_dt_from_state(setmhd_ctx* c, double* dt_out) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const int cur = (int)(c->step % 3);
   double h[6] = {save[0], save[1], save[2], save[3], save[4], save[5]};
+  // the discarded update runs with dt = 0, so a large armed dt cannot drive
+  // it to rho <= 0 and raise an error for a sound state
+  h[cur] = 0.0;
   h[(cur + 1) % 3] = kDtReset;
   HIPCHK(c, hipMemcpyAsync(c->d_scal, h, sizeof h, hipMemcpyHostToDevice, c->stream));
   hipError_t e;

@@ -141,3 +141,59 @@ def test_batch_residual_matches_single():
         c.step(8)
         r_single = c.step_residual()
     assert r_batch > 0 and r_batch == pytest.approx(r_single, rel=1e-9)
+
+
+def test_query_and_download_of_a_slot_in_the_running_step():
+    """A grid admitted before an asynchronous step is part of that step: a
+    query or download issued at once waits for it (state and step count of
+    the finished step, never a torn one)."""
+    case("batch: query / download of a fresh slot while its first step runs")
+    (gA, WA), (gB, WB) = grids()[1], grids()[3]
+    with solver.Context(solver.cartesian_config()) as c:
+        a = c.batch_add(gA, WA)
+        b = c.batch_add(gB, WB)
+        c.batch_step_async(7)
+        cpA, tA = c.batch_download(a)
+        qb = c.batch_query(b)
+        qa = c.batch_query(a)
+        cpB, tB = c.batch_download(b)
+        c.batch_wait()
+    assert qa["steps"] == 7 and qb["steps"] == 7
+    for cp, t, (g, W) in ((cpA, tA, (gA, WA)), (cpB, tB, (gB, WB))):
+        ref, tref = alone(g, W, 7)
+        e = rel_err(cons(cp), cons(ref), "inf")
+        assert np.all(e <= 1e-12), e
+        assert t == pytest.approx(tref, rel=1e-14)
+
+
+def test_step_error_survives_an_overlapping_admission():
+    """A step that drives a density to zero (a CFL number of 50) raises the
+    step's error word; an admission whose upload and first-dt scan run
+    beside that step checks its own word, so batch_wait still reports the
+    step's error."""
+    case("batch: rho error of a running step, admission beside it, error reported by wait")
+    (gA, WA), (gB, WB) = grids()[0], grids()[2]
+    with solver.Context(solver.cartesian_config(cfl=50.0)) as c:
+        c.batch_add(gA, WA)
+        c.batch_step_async(1)
+        c.batch_add(gB, WB)  # its scan runs with dt = 0: no error of its own
+        with pytest.raises(solver.SetMHDError) as e:
+            c.batch_wait()
+        assert e.value.status == solver.ERR_RHO
+
+
+def test_dt_from_state_after_a_large_dt():
+    """dt_from_state's scan discards its update and runs it with dt = 0, so
+    an earlier set_dt far beyond the CFL limit cannot make it report a
+    density error for a sound state."""
+    case("dt_from_state after set_dt(1e3)")
+    g, W = grids()[2]
+    with solver.Context(solver.cartesian_config()) as c:
+        c.upload_structured(g, W)
+        c.set_dt(1.0e3)
+        dt = c.dt_from_state()
+        assert 0.0 < dt < 1.0
+        c.step(3)
+        cp, _ = c.download()
+    ref, _ = alone(g, W, 3)
+    np.testing.assert_array_equal(cp, ref)

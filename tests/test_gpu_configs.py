@@ -1,0 +1,136 @@
+"""The batching and serving workloads at their stated sizes.
+
+Batching: the 64 uneven 2-D grids (128x64 to 1024x512, seeded parameters)
+in one batch for 1000 steps, FP64 and mixed; every grid equals the same grid
+stepped alone through the single-grid path, and the smallest, median and
+largest grids equal the CPU oracle over a step count the CPU can afford.
+
+Serving: a seeded Poisson stream of heliosphere solves (60 % 2-D, 40 % 3-D)
+run to relative density residual <= 1e-6 or 5000 steps; every result of the
+continuous-batching server equals the one-at-a-time result, and the 2-D
+requests, capped at a step count the CPU can afford, equal the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_err
+from set_amd import mesh, serve, solver, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def cons(cp):
+    return mesh.prim_to_cons(cp[:, :8])
+
+
+def alone(g, W, nsteps, precision):
+    with solver.Context(solver.cartesian_config(precision=precision)) as c:
+        c.upload_structured(g, W)
+        c.dt_from_state()
+        c.step(nsteps)
+        cp, t = c.download()
+    return cp, t
+
+
+def oracle_run(g, W, nsteps):
+    s = oracle.Solver(mesh.face_mesh(g), W, oracle.cartesian_config())
+    s.dt_next = s.dt_scan()
+    assert s.run(nsteps) == 0
+    return s
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch64_1000_steps_equal_alone(precision):
+    grids = workloads.batch_grids(64)
+    case(f"batch of 64 2-D grids ({sum(g.n_cell for g, _ in grids)} cells), 1000 steps, "
+         f"precision={precision}, autotuned, vs each grid alone")
+    with solver.Context(solver.cartesian_config(precision=precision)) as c:
+        slots = [c.batch_add(g, W) for g, W in grids]
+        c.batch_autotune(probe_steps=3)
+        c.batch_step(1000)
+        res = [c.batch_download(s) for s in slots]
+        q = [c.batch_query(s) for s in slots]
+    worst = 0.0
+    for (g, W), (cp, t), qq in zip(grids, res, q):
+        assert qq["steps"] == 1000
+        ref, tref = alone(g, W, 1000, precision)
+        e = rel_err(cons(cp), cons(ref), "inf")
+        worst = max(worst, e.max())
+        assert t == pytest.approx(tref, rel=1e-13), (g.nx, g.ny)
+        assert np.all(e <= 1e-12), (g.nx, g.ny, e)
+    print("worst rel Linf against alone over 64 grids", worst)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch64_against_oracle(precision):
+    """The smallest, median and largest of the 64 grids, from the batch,
+    against the oracle after 60 steps: FP64 to 1e-10 relative L1 on all 8
+    conserved variables; the mixed form's density to 1e-5 (its bound
+    against FP64)."""
+    grids = workloads.batch_grids(64)
+    order = np.argsort([g.n_cell for g, _ in grids], kind="stable")
+    pick = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
+    case(f"batch of 64 2-D grids, 60 steps, precision={precision}: grids {pick} vs oracle")
+    with solver.Context(solver.cartesian_config(precision=precision)) as c:
+        slots = [c.batch_add(g, W) for g, W in grids]
+        c.batch_step(60)
+        res = {i: c.batch_download(slots[i]) for i in pick}
+    for i in pick:
+        g, W = grids[i]
+        s = oracle_run(g, W, 60)
+        cp, t = res[i]
+        e = rel_err(cons(cp), cons(s.W), "l1")
+        print(g.nx, g.ny, "rel L1 per var", e)
+        assert t == pytest.approx(s.t, rel=1e-12)
+        if precision == 0:
+            assert np.all(e <= 1e-10), e
+        else:
+            assert e[0] <= 1e-5, e
+
+
+def test_serving_stream_equal_alone():
+    """12 requests of the serving stream under its rule (residual <= 1e-6 or
+    5000 steps): each batched result equals the one-at-a-time result."""
+    reqs = workloads.serving_requests(12)
+    states = [r.state() for r in reqs]
+    case("serving stream: 12 requests (" + ", ".join(
+        f"{r.grid.nx}x{r.grid.ny}x{r.grid.nz}" for r in reqs) + "), tol 1e-6, max 5000 steps")
+    done_b, stats_b = serve.Server().run(reqs, states, keep_results=True)
+    done_s, stats_s = serve.run_sequential(reqs, states, keep_results=True)
+    assert stats_b["solves"] == stats_s["solves"] == len(reqs)
+    by_b = {d["rid"]: d for d in done_b}
+    for d in done_s:
+        b = by_b[d["rid"]]
+        assert b["steps"] == d["steps"] and b["converged"] == d["converged"]
+        e = rel_err(cons(b["result"]), cons(d["result"]), "inf")
+        assert np.all(e <= 1e-12), (d["rid"], e)
+        assert b["t"] == pytest.approx(d["t"], rel=1e-13)
+    print("batched", {k: v for k, v in stats_b.items() if k != "seconds_in"})
+    print("sequential", stats_s)
+
+
+def test_serving_2d_requests_against_oracle():
+    """The 2-D requests of the same stream, served together with the 3-D ones
+    but capped at 150 steps, against the oracle after 150 steps (1e-10
+    relative L1, all 8 conserved variables)."""
+    reqs = workloads.serving_requests(12)
+    states = [r.state() for r in reqs]
+    case("serving stream: 12 requests capped at 150 steps; 2-D results vs oracle")
+    done, _ = serve.Server(max_steps=150, tol=-1.0).run(reqs, states, keep_results=True)
+    by = {d["rid"]: d for d in done}
+    n2 = 0
+    for r, W in zip(reqs, states):
+        if r.grid.nz > 1:
+            continue
+        d = by[r.rid]
+        assert d["steps"] == 150
+        s = oracle_run(r.grid, W, 150)
+        e = rel_err(cons(d["result"]), cons(s.W), "l1")
+        print(r.grid.nx, r.grid.ny, "rel L1 per var", e)
+        assert np.all(e <= 1e-10), (r.rid, e)
+        n2 += 1
+    assert n2 >= 5

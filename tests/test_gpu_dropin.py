@@ -202,6 +202,46 @@ def test_reorder_bitwise(which):
     assert o["dtn"] == r["dtn"] == b["dtn"]
 
 
+def retyped_helio_mesh(nx, ny):
+    """The half-plane heliosphere mesh with its outer sides retyped: x = -1 a
+    reflecting wall (Gran_type 2 with no second cell, what SET writes for a
+    face whose other cell is not exported, Setka.cpp:1456-1483), y = 1 type
+    1 and x = 1 type 3 without a second cell (copy ghosts); y = 0 stays the
+    axis (type 5)."""
+    fm, W = helio_mesh(nx, ny)
+    out = fm.gran_neighbour[:, 1] < 0
+    n = fm.gran_normal
+    fm.gran_type[out & (n[:, 0] == -1.0)] = mesh.TYPE_INTERIOR
+    fm.gran_type[out & (n[:, 1] == 1.0)] = 1
+    fm.gran_type[out & (n[:, 0] == 1.0)] = 3
+    return fm, W
+
+
+@pytest.mark.parametrize("which", ["reference", "cartesian"])
+def test_wall_and_copy_ghosts(which):
+    """Every boundary type on the GPU against the oracle: the wall, types 1
+    and 3 and the axis, under SET's semantics (TVD, Heun, sources; <= 1e-12
+    for Source_MK's log) and under plain first-order HLLD with Euler (bit
+    for bit, fluxes and next dt included)."""
+    fm, W = retyped_helio_mesh(120, 60)
+    assert set(np.unique(fm.gran_type)) == {1, 2, 3, 5}
+    if which == "reference":
+        ocfg, nsteps, dt0 = oracle.reference_config(), 12, 2.0e-9
+    else:
+        ocfg, nsteps = oracle.cartesian_config(riemann=3, recon=1, time_scheme=1, r_hold=0.1), 40
+        dt0 = oracle.Solver(fm, W, ocfg).dt_scan()
+    case(f"dropin wall + types 1/3/5 {which} 120x60 steps={nsteps}")
+    s, cp, t, dtn, pot = run_both(fm, W, ocfg, nsteps, dt0)
+    if which == "reference":
+        err = np.abs(cp[:, :8] - s.W[:, :8]).max(0) / np.maximum(np.abs(s.W[:, :8]).max(0), 1e-300)
+        assert np.all(err <= 1e-12), err
+        assert t == pytest.approx(s.t, rel=1e-13)
+    else:
+        np.testing.assert_array_equal(cp[:, :8], s.W[:, :8])
+        np.testing.assert_array_equal(pot, s.potok)
+        assert t == s.t and dtn == s.dt_next
+
+
 def test_density_failure_reported():
     case("dropin forced rho<0 -> SETMHD_ERR_RHO")
     g = mesh.grid_2d(40, 20)
@@ -231,6 +271,17 @@ def test_upload_validation():
         W2[5, 0] = 0.0
         with pytest.raises(solver.SetMHDError):
             c.upload_mesh(fm, W2)
+        # Set_Storage compares rho with the default-REAL literal 1e-9
+        # (9.99999972e-10): a density between the two is accepted, as the
+        # Fortran and Python readers accept it
+        W2[5, 0] = 9.999999985e-10
+        c.upload_mesh(fm, W2)
+        for col in (8, 11, 12, 15):  # Cell_par(9), (12), (13), (16) must be >= 0
+            W3 = W.copy()
+            W3[7, col] = -1e-3
+            with pytest.raises(solver.SetMHDError) as e:
+                c.upload_mesh(fm, W3)
+            assert e.value.status == solver.ERR_ARG
 
 
 @pytest.mark.parametrize("mode", ["reference", "cartesian"])

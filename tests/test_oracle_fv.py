@@ -102,12 +102,15 @@ def brio_wu(n, axis, nz=1):
     return g, fm, W, c
 
 
+@pytest.mark.parametrize("recon,scheme", [(0, 0), (1, 1)])
 @pytest.mark.parametrize("riemann", [2, 3])
-def test_shock_tube_axis_permutation(riemann):
+def test_shock_tube_axis_permutation(riemann, recon, scheme):
+    """Brio-Wu along x, y and z: identical profiles (first order with Euler,
+    minmod TVD with Heun)."""
     res = []
     for axis in (0, 1, 2):
         g, fm, W, c = brio_wu(64, axis, nz=3 if axis == 2 else 1)
-        cfg = oracle.cartesian_config(riemann=riemann, r_hold=-1.0)
+        cfg = oracle.cartesian_config(riemann=riemann, recon=recon, time_scheme=scheme, r_hold=-1.0)
         s = oracle.Solver(fm, W, cfg)
         s.dt_next = s.dt_scan()
         assert s.run(30) == 0
