@@ -167,13 +167,13 @@ def test_query_and_download_of_a_slot_in_the_running_step():
 
 
 def test_step_error_survives_an_overlapping_admission():
-    """A step that drives a density to zero (a CFL number of 50) raises the
+    """A step that drives a density to zero (a CFL number of 1e4) raises the
     step's error word; an admission whose upload and first-dt scan run
     beside that step checks its own word, so batch_wait still reports the
     step's error."""
     case("batch: rho error of a running step, admission beside it, error reported by wait")
     (gA, WA), (gB, WB) = grids()[0], grids()[2]
-    with solver.Context(solver.cartesian_config(cfl=50.0)) as c:
+    with solver.Context(solver.cartesian_config(cfl=1.0e4)) as c:
         c.batch_add(gA, WA)
         c.batch_step_async(1)
         c.batch_add(gB, WB)  # its scan runs with dt = 0: no error of its own

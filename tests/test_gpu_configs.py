@@ -85,7 +85,8 @@ def test_batch64_against_oracle(precision):
         cp, t = res[i]
         e = rel_err(cons(cp), cons(s.W), "l1")
         print(g.nx, g.ny, "rel L1 per var", e)
-        assert t == pytest.approx(s.t, rel=1e-12)
+        # the mixed form takes its wave speeds, so its dt, in FP32
+        assert t == pytest.approx(s.t, rel=1e-12 if precision == 0 else 1e-5)
         if precision == 0:
             assert np.all(e <= 1e-10), e
         else:
