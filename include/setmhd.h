@@ -272,6 +272,26 @@ int setmhd_download_wait(setmhd_ctx *ctx, double *cell_par, double *t);
  * (cuf_kernel.cuf:886-906). */
 int setmhd_probe(setmhd_ctx *ctx, int32_t n, const int32_t *cells, double *out, double *t);
 
+/* One forward-Euler step of a structured box held in the caller's device
+ * memory, allocating nothing: no counterpart in the reference; for callers
+ * that cannot allocate device memory (HIP graph capture, tight serving
+ * loops).  d_in and d_out (distinct, on the current device) hold the FP64
+ * conserved state as the structured path keeps it: 8 fields (rho, rho u,
+ * rho v, rho w, E, bx, by, bz with b = B/sqrt(4 pi)) of nx*ny*nz doubles
+ * each, cell c = i + nx*(j + ny*k); box and cfg as for
+ * setmhd_upload_structured (first-order HLLD, forward Euler, no heliosphere
+ * switches).  The step uses time step dt and writes U^{n+1} to d_out;
+ * *dt_next receives the CFL minimum of d_in's faces (the next step's dt
+ * under the reference's lagged rule, cuf_kernel.cuf:789-790) and
+ * *residual ||rho^{n+1}-rho^n||_1 / ||rho^n||_1 (either may be NULL).  Runs
+ * on `stream` (a hipStream_t; NULL: the null stream) and waits for it.
+ * The step's scalars live in a 64-byte static device array shared by every
+ * call on that device: calls that overlap on one device race.  Grids of
+ * more than 2^31-1 cells are rejected (SETMHD_ERR_ARG). */
+int setmhd_step_noalloc(const setmhd_config *cfg, int32_t nx, int32_t ny, int32_t nz,
+                        const double *box /*[6]*/, const double *d_in, double *d_out, double dt,
+                        void *stream, double *dt_next, double *residual);
+
 /* Message of the last failure of this context ("" if none). */
 int setmhd_last_error(const setmhd_ctx *ctx, char *buf, size_t n);
 

@@ -360,14 +360,14 @@ UCfg ucfg(const setmhd_config& k) {
 // tile of block rows or WPB tiles of wave rows), so take the chunk in
 // [1, 32] with the fewest row-times over its rounds, (k + 3) x rounds,
 // among those whose tiles fill half a round (one row when none does).
-int grid_kchunk(const setmhd_ctx* c, int nx, int ny, int nz, bool batch) {
-  if (c->cfg.kchunk > 0) return c->cfg.kchunk;
+int kchunk_for(const setmhd_config& cfg, int n_cu, int nx, int ny, int nz, bool batch) {
+  if (cfg.kchunk > 0) return cfg.kchunk;
   if (batch && nz > 1) return 24;
   if (nz <= 1) {
     if (batch) return 16;
     const bool waves = nx <= kWaveRowsMaxNx;
     const long long nseg = waves ? (nx + WRN - 1) / WRN : (nx + ROWN - 1) / ROWN;
-    const long long slots = (waves ? 2LL * WPB : 2LL) * c->n_cu;
+    const long long slots = (waves ? 2LL * WPB : 2LL) * n_cu;
     int best = 1;
     long long best_cost = -1;
     for (int k = 1; k <= 32; k++) {
@@ -382,24 +382,29 @@ int grid_kchunk(const setmhd_ctx* c, int nx, int ny, int nz, bool batch) {
     return best;
   }
   const long long txy = (long long)((nx + 31) / 32) * ((ny + 7) / 8);
-  const long long kc = (nz * txy + 4LL * c->n_cu - 1) / (4LL * c->n_cu);
+  const long long kc = (nz * txy + 4LL * n_cu - 1) / (4LL * n_cu);
   return (int)std::min(32LL, std::max(8LL, kc));
+}
+
+int grid_kchunk(const setmhd_ctx* c, int nx, int ny, int nz, bool batch) {
+  return kchunk_for(c->cfg, c->n_cu, nx, ny, nz, batch);
 }
 
 // Default z-march tiles of a grid alone: 16x16 in FP64 (256^3: +1.5 %
 // against 32x8); in the mixed form 32x8, whose planes arrive by LDS DMA,
 // where rows have even length (256^3: +3 % against 64x4), else 64x4
-bool mixed_dma_tile(const setmhd_ctx* c) { return c->sg.nx % 2 == 0; }
-int tile_x(const setmhd_ctx* c) {
-  if (c->cfg.tile_x > 0) return c->cfg.tile_x;
-  if (c->cfg.precision == SETMHD_PREC_FP64) return 16;
-  return mixed_dma_tile(c) ? 32 : 64;
+int tile_x_for(const setmhd_config& cfg, int nx) {
+  if (cfg.tile_x > 0) return cfg.tile_x;
+  if (cfg.precision == SETMHD_PREC_FP64) return 16;
+  return nx % 2 == 0 ? 32 : 64;
 }
-int tile_y(const setmhd_ctx* c) {
-  if (c->cfg.tile_y > 0) return c->cfg.tile_y;
-  if (c->cfg.precision == SETMHD_PREC_FP64) return 16;
-  return mixed_dma_tile(c) ? 8 : 4;
+int tile_y_for(const setmhd_config& cfg, int nx) {
+  if (cfg.tile_y > 0) return cfg.tile_y;
+  if (cfg.precision == SETMHD_PREC_FP64) return 16;
+  return nx % 2 == 0 ? 8 : 4;
 }
+int tile_x(const setmhd_ctx* c) { return tile_x_for(c->cfg, c->sg.nx); }
+int tile_y(const setmhd_ctx* c) { return tile_y_for(c->cfg, c->sg.nx); }
 
 // drop-in kernels: the exact build, or the fast one under SETMHD_F_FAST
 template <class... A>
@@ -1127,3 +1132,77 @@ int setmhd_last_error(const setmhd_ctx* c, char* buf, size_t n) {
 }  // extern "C"
 
 #include "batch_impl.inc"
+
+// ---- one structured step without allocation (setmhd_step_noalloc) --------
+// The step's scalars -- dt slots, time, density-residual sums and the error
+// word -- live in `scratch`, one copy per device, 64 bytes.  Host code never
+// names the array in a device call: it uses the address hipGetSymbolAddress
+// returns on each device, taken once per device with that device current.
+namespace {
+
+constexpr int kNoallocScratchDoubles = 8;  // [0..2] dt slots, [3] t, [4..5] residual sums, [6] error word
+constexpr int kNoallocMaxDevices = 64;
+static __device__ __align__(256) double scratch[kNoallocScratchDoubles];
+
+double* g_scratch_ptr[kNoallocMaxDevices];
+hipError_t g_scratch_err[kNoallocMaxDevices];
+std::once_flag g_scratch_once[kNoallocMaxDevices];
+
+// The device address of this device's scratch (device `dev` current).
+double* scratch_on(int dev, hipError_t* err) {
+  std::call_once(g_scratch_once[dev], [dev]() {
+    void* p = nullptr;
+    g_scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_ptr[dev] = g_scratch_err[dev] == hipSuccess ? static_cast<double*>(p) : nullptr;
+  });
+  *err = g_scratch_err[dev];
+  return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+extern "C" int setmhd_step_noalloc(const setmhd_config* cfg, int32_t nx, int32_t ny, int32_t nz,
+                                   const double* box, const double* d_in, double* d_out, double dt,
+                                   void* stream, double* dt_next, double* residual) {
+  if (!cfg || !box || !d_in || !d_out || d_in == d_out) return SETMHD_ERR_ARG;
+  if (nx < 1 || ny < 1 || nz < 1 || (long long)nx * ny * nz > (1LL << 31) - 1) return SETMHD_ERR_ARG;
+  if (!(dt > 0.0) || cfg->precision < 0 || cfg->precision > 1 || cfg->kchunk < 0) return SETMHD_ERR_ARG;
+  if (cfg->riemann != SETMHD_RIEMANN_HLLD || cfg->recon != 0 || cfg->time_scheme != SETMHD_TIME_EULER ||
+      (cfg->flags & (SETMHD_F_ZERO_BW | SETMHD_F_AXISYM | SETMHD_F_SOURCE_MK | SETMHD_F_HELIO_HACKS)))
+    return SETMHD_ERR_UNSUPPORTED;
+  int dev = 0, n_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return SETMHD_ERR_HIP;
+  if (dev < 0 || dev >= kNoallocMaxDevices) return SETMHD_ERR_UNSUPPORTED;
+  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n_cu < 1)
+    return SETMHD_ERR_HIP;
+  hipError_t e;
+  double* sp = scratch_on(dev, &e);
+  if (e != hipSuccess || !sp) return SETMHD_ERR_HIP;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // dt of this step in slot 0, the face minimum reduced into slot 1; time,
+  // residual sums and error word cleared (a copy, not a memset node)
+  const double hin[kNoallocScratchDoubles] = {dt, kDtReset, kDtReset, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double h[kNoallocScratchDoubles];
+  if (hipMemcpyAsync(sp, hin, sizeof hin, hipMemcpyHostToDevice, s) != hipSuccess) return SETMHD_ERR_HIP;
+  FusedArgs a{};
+  a.g = make_sgrid(*cfg, nx, ny, nz, box);
+  a.Uf = d_in;
+  a.Un = d_in;
+  a.Uo = d_out;
+  a.dt = DtRing{sp, sp + 3};
+  a.cur = 0;
+  a.kchunk = kchunk_for(*cfg, n_cu, nx, ny, nz, false);
+  a.resid = sp + 4;
+  a.err = reinterpret_cast<int*>(sp + 6);
+  e = launch_fused(s, a, kModeEuler, cfg->precision, tile_x_for(*cfg, nx), tile_y_for(*cfg, nx));
+  if (e != hipSuccess) return e == hipErrorInvalidValue ? SETMHD_ERR_ARG : SETMHD_ERR_HIP;
+  if (hipMemcpyAsync(h, sp, sizeof h, hipMemcpyDeviceToHost, s) != hipSuccess) return SETMHD_ERR_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess) return SETMHD_ERR_HIP;
+  if (dt_next) *dt_next = h[1];
+  if (residual) *residual = h[5] > 0.0 ? h[4] / h[5] : 0.0;
+  int err = 0;
+  std::memcpy(&err, &h[6], sizeof err);
+  if (err & kErrRho) return SETMHD_ERR_RHO;
+  if (err & kErrRegion) return SETMHD_ERR_REGION;
+  return SETMHD_OK;
+}

@@ -42,7 +42,7 @@ EXPORTS = (
     "setmhd_batch_add", "setmhd_batch_remove", "setmhd_batch_step", "setmhd_batch_step_async",
     "setmhd_batch_poll", "setmhd_batch_wait", "setmhd_batch_query",
     "setmhd_batch_download", "setmhd_batch_autotune", "setmhd_batch_set_launch", "setmhd_mem_info",
-    "setmhd_mesh_order",
+    "setmhd_mesh_order", "setmhd_step_noalloc",
 )
 
 
@@ -109,6 +109,8 @@ def load():
                                         ctypes.POINTER(I32)]
     L.setmhd_mem_info.argtypes = [P, I64P, I64P]
     L.setmhd_mesh_order.argtypes = [I32, I32, PD, PD, PI, PI, PI]
+    L.setmhd_step_noalloc.argtypes = [ctypes.POINTER(Config), I32, I32, I32, PD, ctypes.c_void_p,
+                                      ctypes.c_void_p, D, ctypes.c_void_p, PD, PD]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is not None and name not in ("setmhd_config_reference",
@@ -151,6 +153,21 @@ def mesh_order(fm):
     if st:
         raise SetMHDError(st, "mesh_order: first face neighbour out of range or empty mesh")
     return cp, gp
+
+
+def step_noalloc(config, grid, d_in, d_out, dt, stream=None):
+    """setmhd_step_noalloc: one forward-Euler step of a structured box whose
+    conserved state (8 fields of grid.n_cell doubles) the caller keeps in
+    device memory at address d_in; U^{n+1} goes to d_out.  Returns
+    (dt_next, residual)."""
+    box = np.array([grid.x0, grid.x1, grid.y0, grid.y1, grid.z0, grid.z1], np.float64)
+    dtn, res = D(), D()
+    st = load().setmhd_step_noalloc(ctypes.byref(config), grid.nx, grid.ny, grid.nz, _pd(box),
+                                    ctypes.c_void_p(d_in), ctypes.c_void_p(d_out), float(dt),
+                                    ctypes.c_void_p(stream), ctypes.byref(dtn), ctypes.byref(res))
+    if st != OK:
+        raise SetMHDError(st, "step_noalloc failed")
+    return dtn.value, res.value
 
 
 def _pd(a):

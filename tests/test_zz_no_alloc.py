@@ -1,0 +1,93 @@
+"""setmhd_step_noalloc against the context path (setmhd_upload_structured +
+setmhd_step), on two calls in a row, the second with a smaller grid: the
+same step to 1e-12 relative L-inf per conserved variable, the same next dt
+and density residual.  The caller's device buffers come from hipMalloc of
+the HIP runtime the library links (not torch's)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from set_amd import mesh, solver
+
+pytestmark = pytest.mark.gpu
+
+
+def hip():
+    solver.load()  # the library's runtime first, so the same libamdhip64 is used
+    h = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so.7")
+    h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    h.hipFree.argtypes = [ctypes.c_void_p]
+    h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    return h
+
+
+H2D, D2H = 1, 2
+
+
+class DevBuf:
+    def __init__(self, h, nbytes):
+        self.h, self.p = h, ctypes.c_void_p()
+        assert h.hipMalloc(ctypes.byref(self.p), nbytes) == 0
+        self.n = nbytes
+
+    def put(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes == self.n
+        assert self.h.hipMemcpy(self.p, a.ctypes.data, self.n, H2D) == 0
+
+    def get(self, shape):
+        a = np.empty(shape)
+        assert a.nbytes == self.n
+        assert self.h.hipMemcpy(a.ctypes.data, self.p, self.n, D2H) == 0
+        return a
+
+    def free(self):
+        self.h.hipFree(self.p)
+
+
+def context_step(g, W):
+    with solver.Context(solver.cartesian_config()) as c:
+        c.upload_structured(g, W)
+        dt = c.dt_from_state()
+        res = c.step_residual()
+        cp, t = c.download()
+        _, dtn = c.sync()
+    return dt, mesh.prim_to_cons(cp[:, :8]), dtn, res, t
+
+
+def test_step_noalloc_matches_context_step():
+    h = hip()
+    cfg = solver.cartesian_config()
+    for g in (mesh.grid_3d(48, 40, 36), mesh.grid_2d(300, 150)):  # the second call: fewer cells
+        print(f"[case] step_noalloc {g.nx}x{g.ny}x{g.nz} vs context step", flush=True)
+        W = mesh.heliosphere_state(g, wind=mesh.Wind(r_in=0.3))
+        dt, ref, dtn_ref, res_ref, t_ref = context_step(g, W)
+        U0 = np.ascontiguousarray(mesh.prim_to_cons(W[:, :8]).T)  # 8 fields of n_cell doubles
+        a, b = DevBuf(h, U0.nbytes), DevBuf(h, U0.nbytes)
+        try:
+            a.put(U0)
+            dtn, res = solver.step_noalloc(cfg, g, a.p.value, b.p.value, dt)
+            U1 = b.get((8, g.n_cell)).T
+        finally:
+            a.free()
+            b.free()
+        e = rel_err(U1, ref, "inf")
+        print("rel Linf per var", e, "dt_next", dtn, dtn_ref, "residual", res, res_ref)
+        assert np.all(e <= 1e-12), e
+        assert t_ref == dt
+        assert dtn == pytest.approx(dtn_ref, rel=1e-13)
+        assert res == pytest.approx(res_ref, rel=1e-12)
+
+
+def test_step_noalloc_rejects_oversized_grid():
+    h = hip()
+    buf = DevBuf(h, 64)
+    try:
+        g = mesh.Grid(2048, 2048, 1024)  # 2^32 cells: beyond the 32-bit cell offsets
+        with pytest.raises(solver.SetMHDError) as e:
+            solver.step_noalloc(solver.cartesian_config(), g, buf.p.value, buf.p.value + 32, 1e-3)
+        assert e.value.status == solver.ERR_ARG
+    finally:
+        buf.free()
