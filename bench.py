@@ -48,10 +48,14 @@ def from_profiles(name, key):
     return None
 
 
-def run_form(g, W, precision, time_scheme, steps, warmup, keep=False):
-    """One form on grid g from Cell_par W.  keep: also return the final
-    conserved state (key "_state", for the mixed form's error table)."""
-    from set_amd import mesh, solver
+def run_form(g, W, precision, time_scheme, steps, warmup, spin_steps=0):
+    """One form on grid g from Cell_par W: `warmup` untimed steps, then
+    `spin_steps` more untimed steps (a device spin-up, so that the timed
+    steps do not run while the clocks ramp; the same count for every form),
+    then exactly `steps` timed steps.  The timed region is bracketed by host
+    clocks and device syncs (value) and, on the library's stream, by HIP
+    events around the same launches (the kernel's average launch duration)."""
+    from set_amd import solver
     cfg = solver.cartesian_config(cfl=0.4, r_hold=0.1, precision=precision,
                                   time_scheme=time_scheme)
     with solver.Context(cfg) as c:
@@ -60,19 +64,22 @@ def run_form(g, W, precision, time_scheme, steps, warmup, keep=False):
         c.sync()
         t_up = time.perf_counter() - t0
         c.dt_from_state()
+        t0 = time.perf_counter()
         if warmup:
             c.step(warmup)
         c.sync()
+        t_warm = time.perf_counter() - t0
+        if spin_steps:
+            c.step(spin_steps)
+            c.sync()
         t0 = time.perf_counter()
-        c.step(steps)
+        ms_ev = c.step_timed(steps)  # HIP events on the library's stream, these launches
         c.sync()
         wall = time.perf_counter() - t0
-        ms_ev = c.step_timed(steps)  # kernel time on the library's stream
         t_sim, dt = c.sync()
         t0 = time.perf_counter()
         cp, _ = c.download()
         t_down = time.perf_counter() - t0
-    state = mesh.prim_to_cons(cp[:, :8]) if keep else None
     del cp
     launches = steps * (2 if time_scheme == 1 else 1)
     bytes_per_cell = 320 if time_scheme == 1 else 128
@@ -81,6 +88,7 @@ def run_form(g, W, precision, time_scheme, steps, warmup, keep=False):
         "ms_per_step": 1e3 * wall / steps,
         "kernel_ms_per_launch": ms_ev / launches,
         "achieved_GBs": g.n_cell * bytes_per_cell * steps / (ms_ev * 1e-3) / 1e9,
+        "achieved_GBs_wall": g.n_cell * bytes_per_cell * steps / wall / 1e9,
         # the same steps with one upload and one download of Cell_par
         # (host buffers over PCIe, conversion kernels included)
         "upload_s": t_up,
@@ -88,9 +96,25 @@ def run_form(g, W, precision, time_scheme, steps, warmup, keep=False):
         "pcie_inclusive_value": g.n_cell * steps / (wall + t_up + t_down),
         "t_sim": t_sim,
         "dt": dt,
-        "steps_run": warmup + 2 * steps,
-        "_state": state,
+        "warmup_s": t_warm,
+        "steps_run": warmup + spin_steps + steps,
     }
+
+
+def precision_error(g, W, steps):
+    """The mixed form against FP64 after `steps` steps from the same initial
+    state (untimed runs): per-variable relative L1 and L-inf errors."""
+    from set_amd import mesh, solver
+    out = []
+    for precision in (0, 1):
+        with solver.Context(solver.cartesian_config(cfl=0.4, r_hold=0.1, precision=precision)) as c:
+            c.upload_structured(g, W)
+            c.dt_from_state()
+            c.step(steps)
+            cp, _ = c.download()
+        out.append(mesh.prim_to_cons(cp[:, :8]))
+        del cp
+    return {"steps": steps, **error_table(out[1], out[0])}
 
 
 def error_table(a, b):
@@ -140,13 +164,29 @@ def batch_form(precision, steps, seq_steps):
 
 
 def serving_form(n_requests, max_steps, precision=0):
+    """The serving stream through the continuous-batching server and one at a
+    time.  work_bound: both modes' cell-update rates over the same work.
+    The one-at-a-time rate is over its busy time (the sum of its solves'
+    durations), so it is what it reaches however dense the stream; while
+    both modes are bound by the work rather than the arrivals, the
+    solves/hour ratio is the ratio of the two rates."""
     from set_amd import serve, workloads
     reqs = workloads.serving_requests(n_requests)
     states = [r.state() for r in reqs]
     _, b = serve.Server(precision=precision, max_steps=max_steps).run(reqs, states)
-    _, q = serve.run_sequential(reqs, states, precision=precision, max_steps=max_steps)
+    done_q, q = serve.run_sequential(reqs, states, precision=precision, max_steps=max_steps)
+    busy_q = sum(d["done"] - d["admitted"] for d in done_q)
+    work = b["cell_updates"]
+    work_3d = sum(d["cells"] * d["steps"] for d, r in zip(sorted(done_q, key=lambda d: d["rid"]),
+                                                            sorted(reqs, key=lambda r: r.rid))
+                  if r.grid.nz > 1)
     return {"requests": n_requests, "max_steps": max_steps, "batched": b, "one_at_a_time": q,
-            "solves_per_hour_ratio": b["solves_per_hour"] / q["solves_per_hour"]}
+            "solves_per_hour_ratio": b["solves_per_hour"] / q["solves_per_hour"],
+            "work_bound": {"cell_updates": work, "frac_3d": work_3d / work if work else None,
+                           "last_arrival_s": max(r.arrival for r in reqs),
+                           "batched_rate": work / b["wall_s"],
+                           "one_at_a_time_busy_rate": q["cell_updates"] / busy_q,
+                           "rate_ratio": (work / b["wall_s"]) / (q["cell_updates"] / busy_q)}}
 
 
 def dropin_form(nx, ny, steps):
@@ -254,10 +294,14 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--forms", default="fp64,mixed,heun,batch64,serving,dropin,facelist")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--spin-s", type=float, default=0.3,
+                    help="untimed device spin-up (seconds of FP64 steps) after the warm-up steps")
+    ap.add_argument("--error-steps", type=int, default=200,
+                    help="steps of the untimed FP64 and mixed runs behind the precision error table")
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--batch-steps", type=int, default=1000)
-    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-max-steps", type=int, default=5000)
     ap.add_argument("--dropin-mesh", default="2000x1000")
     ap.add_argument("--dropin-steps", type=int, default=20)
@@ -285,18 +329,24 @@ def main():
     W = mesh.heliosphere_state(g)
     forms = {}
     wanted = args.forms.split(",")
-    fp = run_form(g, W, 0, 0, args.steps, args.warmup, keep="mixed" in wanted)
+    # spin-up: about 0.3 s of device work before the first timed step, the
+    # same step count for every form (from the FP64 warm-up's rate)
+    spin = 0
+    if args.spin_s > 0:
+        probe = run_form(g, W, 0, 0, 4, args.warmup)
+        spin = min(4000, int(args.spin_s / (probe["ms_per_step"] * 1e-3)))
+    fp = run_form(g, W, 0, 0, args.steps, args.warmup, spin)
     forms["fp64_euler"] = fp
     if "mixed" in wanted:
-        mx = run_form(g, W, 1, 0, args.steps, args.warmup, keep=True)
+        mx = run_form(g, W, 1, 0, args.steps, args.warmup, spin)
         forms["mixed_euler"] = mx
         mx["speedup_vs_fp64"] = mx["value"] / fp["value"]
-        # the precision form's error against FP64 after the same steps
-        mx["error_vs_fp64"] = {"steps": mx["steps_run"], **error_table(mx["_state"], fp["_state"])}
+        mx["kernel_speedup_vs_fp64"] = fp["kernel_ms_per_launch"] / mx["kernel_ms_per_launch"]
+        # the precision form's error against FP64 after the config's 200 steps
+        mx["error_vs_fp64"] = precision_error(g, W, args.error_steps)
     if "heun" in wanted:
-        forms["fp64_heun"] = run_form(g, W, 0, 1, args.steps, args.warmup)
+        forms["fp64_heun"] = run_form(g, W, 0, 1, args.steps, args.warmup, spin)
     for f in forms.values():
-        f.pop("_state", None)
         f["hbm_frac"] = f["achieved_GBs"] / PEAK_HBM_GBS
     del W
     for name, fn in (("batch64_fp64", lambda: batch_form(0, args.batch_steps, 20)),
@@ -317,14 +367,24 @@ def main():
 
     traffic = from_profiles(f"fused_fp64_euler_{n}cube", "traffic_bytes_per_launch")
     achieved = fp["achieved_GBs"]
+    # achieved: 128 algorithmic bytes per cell-update x cells per launch over
+    # the kernel's average launch duration, from HIP events around the timed
+    # steps themselves; frac_wall: the same bytes over the timed region's
+    # wall clock (launch gaps included).  traffic: HBM bytes per launch from
+    # the committed rocprofv3 FETCH_SIZE / WRITE_SIZE profile of this kernel
+    # at this size (a profile-derived figure, not measured by this run).
     roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": achieved / PEAK_HBM_GBS, "traffic": traffic}
+                "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                "frac_wall": fp["achieved_GBs_wall"] / PEAK_HBM_GBS,
+                "kernel": "k_fused<double,16,16,3,Euler> (one launch per step)",
+                "traffic_source": f"profiles/fused_fp64_euler_{n}cube.txt" if traffic else None}
     # FP64 flops of the same kernel (VALU counters of the committed profile)
     fl = from_profiles(f"fused_fp64_euler_{n}cube", "fp64_flops_per_cell_update")
     if fl:
         tf = fl * g.n_cell / (fp["kernel_ms_per_launch"] * 1e-3) / 1e12
         fp["compute"] = {"flops_per_cell_update": fl, "achieved": tf, "peak": PEAK_FP64_TFLOPS,
-                         "unit": "TFLOP/s (FP64 vector)", "frac": tf / PEAK_FP64_TFLOPS}
+                         "unit": "TFLOP/s (FP64 vector)", "frac": tf / PEAK_FP64_TFLOPS,
+                         "flops_source": f"profiles/fused_fp64_euler_{n}cube.txt (VALU counters)"}
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -336,7 +396,8 @@ def main():
         "data": "synthetic (seeded heliosphere initial state)",
         "config": {"workload": f"heliosphere 3-D {n}^3 ideal MHD, first-order HLLD, forward "
                                "Euler, CFL 0.4, box [-1,1]^3", "cells": g.n_cell,
-                   "bytes_per_cell_update": 128, "parallelism": "single GPU"},
+                   "bytes_per_cell_update": 128, "parallelism": "single GPU",
+                   "spin_up_steps": spin},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "forms": forms,

@@ -231,6 +231,19 @@ int setmhd_batch_query(setmhd_ctx *ctx, int32_t slot, int64_t *steps,
 int setmhd_batch_download(setmhd_ctx *ctx, int32_t slot, double *cell_par,
                           double *t);
 
+/* Queues a copy of a slot's Cell_par (16 doubles per cell) and time to
+ * pinned host memory and returns at once (after the running step, if the
+ * slot takes part in it); *ticket names the copy.  The copy runs on the
+ * context's second stream, beside steps queued meanwhile; a retired slot's
+ * memory is freed only after its queued copies have run. */
+int setmhd_batch_download_async(setmhd_ctx *ctx, int32_t slot, int32_t *ticket);
+
+/* Collects a queued copy.  wait = 0: returns at once, *done = 0 while the
+ * copy runs; otherwise waits for it.  When done (*done = 1) cell_par and *t
+ * (may be NULL) hold the slot's state and time and the ticket is free. */
+int setmhd_batch_download_collect(setmhd_ctx *ctx, int32_t ticket, int32_t wait, double *cell_par,
+                                  double *t, int32_t *done);
+
 /* Times probe_steps steps of the admitted batch for each launch shape
  * (persistent grids of 1, 2, 4, 8, 64 blocks per CU, or one block per tile,
  * reported as 0) and, for the 3-D grids, each tile shape (32x8, 16x16,

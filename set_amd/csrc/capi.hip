@@ -107,6 +107,18 @@ struct setmhd_ctx {
   bool snap_pending = false;
   hipEvent_t ev_snapc = nullptr, ev_snap = nullptr;
   int n_cu = 256;
+  // setmhd_batch_download_async: queued copies of retired (or idle) slots to
+  // pinned host memory on the side stream, one per ticket; pinned buffers
+  // are kept for reuse when a ticket is collected
+  struct Ticket {
+    bool used = false;
+    long long nc = 0;
+    double t = 0.0;
+    double* host = nullptr;  // pinned, cap doubles
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+  };
+  std::vector<Ticket> tickets;
 };
 
 // per-slot device scalars of a batch
@@ -154,6 +166,11 @@ void free_batch(setmhd_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->side) hipStreamSynchronize(c->side);
   free_graphs(c);
+  for (auto& tk : c->tickets) {
+    if (tk.host) hipHostFree(tk.host);
+    if (tk.ev) hipEventDestroy(tk.ev);
+  }
+  c->tickets.clear();
   for (auto& sl : c->slots) {
     for (double* p : {sl.buf[0], sl.buf[1], sl.h16})
       if (p) hipFree(p);

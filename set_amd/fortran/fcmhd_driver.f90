@@ -22,8 +22,15 @@
 ! (SETMHD_F_REORDER); files, probes and dumps keep the input's numbering and
 ! the results are the same bits.
 !
+! Restart: the reference resumes from Fill_data's time and state with the
+! initial dt 2e-9 (cuf_kernel.cuf:758); so does this driver.  It also writes
+! output.bin.dt (the time and the lagged dt the next step would use, two
+! REAL(8)), and with 'continue' a restart reads restart.bin.dt and takes that
+! dt, so N steps, a restart and M steps give the bits of N+M steps.
+!
 ! usage: fcmhd_driver input.bin output.bin all_step [print_step] [restart.bin|-]
-!                     [cartesian|reference] [probes.txt|-] [dump|-] [reorder]
+!                     [cartesian|reference] [probes.txt|-] [dump|-] [reorder|-]
+!                     [continue]
 program fcmhd_driver
   use iso_c_binding
   use setmhd
@@ -42,7 +49,8 @@ program fcmhd_driver
   integer(4), allocatable :: probes(:)
   real(8), allocatable :: pv(:, :)
   logical :: dump
-  logical :: cartesian, reorder
+  logical :: cartesian, reorder, cont
+  real(8) :: t_dt, dt_start
   type(setmhd_config) :: cfg
   type(c_ptr) :: ctx
   integer(c_int) :: st
@@ -82,6 +90,11 @@ program fcmhd_driver
   if (nargs >= 9) then
     call get_command_argument(9, arg)
     reorder = trim(arg) == 'reorder'
+  end if
+  cont = .false.
+  if (nargs >= 10) then
+    call get_command_argument(10, arg)
+    cont = trim(arg) == 'continue'
   end if
   nprobe = 0
   if (len_trim(fprobes) > 0) then
@@ -140,6 +153,16 @@ program fcmhd_driver
     close(u)
     if (vv /= 321.0_8) stop 'restart file: sentinel 321 missing'
   end if
+  dt_start = 0.000000002_8   ! time_step2, cuf_kernel.cuf:758
+  if (cont) then
+    if (len_trim(frestart) == 0) stop 'continue needs a restart file'
+    open(newunit=u, file=trim(frestart)//'.dt', form='unformatted', access='stream', action='read', &
+         status='old')
+    read(u) t_dt
+    read(u) dt_start
+    close(u)
+    if (t_dt /= host_time_all) stop 'restart file and its .dt disagree on the time'
+  end if
 
   ! ---- Set_CUDA -------------------------------------------------------------
   if (cartesian) then
@@ -154,7 +177,7 @@ program fcmhd_driver
        host_Cell_Volume, host_Cell_gran, host_Gran_normal, host_Gran_square, host_Gran_center, &
        host_Gran_neighbour, host_Gran_neighbour_TVD, host_Gran_type, host_Gran_POTOK, host_time_all)
   call setmhd_check(ctx, st, 'upload_mesh')
-  st = setmhd_set_dt(ctx, 0.000000002_8)   ! time_step2, cuf_kernel.cuf:758
+  st = setmhd_set_dt(ctx, dt_start)
   call setmhd_check(ctx, st, 'set_dt')
 
   ! ---- CUDA_START_MGD step loop ----------------------------------------------
@@ -184,7 +207,14 @@ program fcmhd_driver
   ! ---- Send_data_to_Host + Save_Storage --------------------------------------
   st = setmhd_download(ctx, host_Cell_par, host_time_all)
   call setmhd_check(ctx, st, 'download')
+  st = setmhd_sync(ctx, t, dt_next)
+  call setmhd_check(ctx, st, 'sync')
   call setmhd_destroy(ctx)
+  open(newunit=u, file=trim(fout)//'.dt', form='unformatted', access='stream', action='write', &
+       status='replace')
+  write(u) t
+  write(u) dt_next
+  close(u)
   open(newunit=u, file=trim(fout), form='unformatted', access='stream', action='write', status='replace')
   write(u) host_time_all
   write(u) host_Cell_par

@@ -12,8 +12,10 @@ steps it
     covers their state plus a reserve (uploads overlap the running chunk),
   * checks each request's relative density residual
     ||rho^{n+1} - rho^n||_1 / ||rho^n||_1 of its last step, and
-  * retires a request when the residual is <= tol or it reached max_steps,
-    downloading its state as the result while the next chunk runs.
+  * retires a request when the residual is <= tol or it reached max_steps:
+    its state is copied to pinned host memory on the library's second stream
+    while the next chunk runs, and collected by polling (the loop never
+    blocks on a result).
 
 A request that arrives while others are on the GPU therefore starts at the
 next chunk boundary instead of waiting for the running ones to finish.
@@ -56,9 +58,11 @@ class _Lane:
         self.in_flight = False
         self.active_cells = 0
         self.tuned = False
+        self.downloads = []  # [ticket, record] of results being copied to the host
+        self.retired_held = False  # retired requests' memory not yet freed
 
     def busy(self):
-        return bool(self.pending or self.active or self.in_flight)
+        return bool(self.pending or self.active or self.in_flight or self.downloads)
 
 
 class Server:
@@ -129,6 +133,22 @@ class Server:
                 retiring.append((slot, i, admitted, steps, q["residual"], conv))
         return retiring
 
+    def _collect(self, lane, done, clock, t0, spent, keep):
+        """Collects the results whose copies have landed (never blocks)."""
+        got = False
+        for item in list(lane.downloads):
+            ticket, rec = item
+            ta = clock()
+            r = lane.ctx.batch_download_collect(ticket, wait=False)
+            spent["retire_collect"] += clock() - ta
+            if r is None:
+                continue
+            cp, t = r
+            lane.downloads.remove(item)
+            done.append(dict(rec, done=clock() - t0, t=t, result=cp if keep else None))
+            got = True
+        return got
+
     def _launch(self, lane, spent, clock):
         if not lane.tuned and len(lane.active) > 1:
             ta = clock()
@@ -167,10 +187,13 @@ class Server:
                 progressed = False
                 for lane in lanes:
                     free = self._admit(lane, requests, states, clock() - t0, free, clock, t0, spent)
+                    if self._collect(lane, done, clock, t0, spent, keep_results):
+                        progressed = True
                     retiring = []
                     if lane.in_flight:
                         # block only when nothing else can need the host
-                        alone = all(not o.busy() for o in lanes if o is not lane) and not lane.pending
+                        alone = (all(not o.busy() for o in lanes if o is not lane) and not lane.pending
+                                 and not lane.downloads)
                         if not alone and not lane.ctx.batch_poll():
                             continue
                         retiring = self._complete(lane, requests, spent, clock)
@@ -180,21 +203,23 @@ class Server:
                         progressed = True
                     for slot, i, admitted, steps, res, conv in retiring:
                         ta = clock()
-                        cp, t = lane.ctx.batch_download(slot)  # overlaps the chunk just started
-                        spent["retire"] += clock() - ta
-                        done.append({"rid": requests[i].rid, "arrival": requests[i].arrival,
-                                     "admitted": admitted, "done": clock() - t0, "steps": steps,
-                                     "residual": res, "converged": conv,
-                                     "cells": requests[i].cells, "t": t,
-                                     "result": cp if keep_results else None})
-                    if not lane.active and not lane.in_flight and retiring:
+                        ticket = lane.ctx.batch_download_async(slot)  # beside the chunk just started
+                        spent["retire_queue"] += clock() - ta
+                        lane.downloads.append([ticket, {
+                            "rid": requests[i].rid, "arrival": requests[i].arrival,
+                            "admitted": admitted, "steps": steps, "residual": res, "converged": conv,
+                            "cells": requests[i].cells}])
+                    if not lane.active and not lane.in_flight and not lane.downloads and lane.retired_held:
                         lane.ctx.batch_wait()  # frees the retired requests' memory
+                        lane.retired_held = False
+                    if retiring:
+                        lane.retired_held = True
                 if progressed:
                     continue
                 now = clock() - t0
                 arrivals = [requests[lane.pending[0]].arrival for lane in lanes if lane.pending]
                 nxt = min(arrivals) - now if arrivals else None
-                if any(lane.in_flight for lane in lanes):
+                if any(lane.in_flight or lane.downloads for lane in lanes):
                     sleep(min(nxt, self.poll_s) if nxt is not None and nxt > 0.0 else self.poll_s)
                 elif nxt is not None and nxt > 0.0:
                     sleep(nxt)

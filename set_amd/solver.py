@@ -42,7 +42,8 @@ EXPORTS = (
     "setmhd_batch_add", "setmhd_batch_remove", "setmhd_batch_step", "setmhd_batch_step_async",
     "setmhd_batch_poll", "setmhd_batch_wait", "setmhd_batch_query",
     "setmhd_batch_download", "setmhd_batch_autotune", "setmhd_batch_set_launch", "setmhd_mem_info",
-    "setmhd_mesh_order", "setmhd_step_noalloc",
+    "setmhd_mesh_order", "setmhd_step_noalloc", "setmhd_batch_download_async",
+    "setmhd_batch_download_collect",
 )
 
 
@@ -105,6 +106,8 @@ def load():
     L.setmhd_batch_poll.argtypes = [P, ctypes.POINTER(I32)]
     L.setmhd_batch_query.argtypes = [P, I32, I64P, PD, PD, PD]
     L.setmhd_batch_download.argtypes = [P, I32, PD, PD]
+    L.setmhd_batch_download_async.argtypes = [P, I32, ctypes.POINTER(I32)]
+    L.setmhd_batch_download_collect.argtypes = [P, I32, I32, PD, PD, ctypes.POINTER(I32)]
     L.setmhd_batch_autotune.argtypes = [P, I32, ctypes.POINTER(I32), ctypes.POINTER(I32),
                                         ctypes.POINTER(I32)]
     L.setmhd_mem_info.argtypes = [P, I64P, I64P]
@@ -343,6 +346,26 @@ class Context:
         cp = np.zeros((self._slot_cells[slot], 16))
         t = D()
         self._check(self._L.setmhd_batch_download(self._h, int(slot), _pd(cp), ctypes.byref(t)))
+        return cp, t.value
+
+    def batch_download_async(self, slot):
+        """Queues a copy of the slot's state to pinned host memory; returns a
+        ticket for batch_download_collect."""
+        k = I32()
+        self._check(self._L.setmhd_batch_download_async(self._h, int(slot), ctypes.byref(k)))
+        self._tickets = getattr(self, "_tickets", {})
+        self._tickets[k.value] = self._slot_cells[slot]
+        return k.value
+
+    def batch_download_collect(self, ticket, wait=True):
+        """(Cell_par, t) of a queued copy, or None while it runs (wait=False)."""
+        cp = np.zeros((self._tickets[ticket], 16))
+        t, done = D(), I32()
+        self._check(self._L.setmhd_batch_download_collect(self._h, int(ticket), int(bool(wait)), _pd(cp),
+                                                          ctypes.byref(t), ctypes.byref(done)))
+        if not done.value:
+            return None
+        del self._tickets[ticket]
         return cp, t.value
 
     def batch_set_launch(self, tile=(32, 8), blocks_per_cu=0, dim=0):

@@ -82,6 +82,26 @@ class ModelContext:
     def batch_download(self, s):
         return np.zeros(((self.slots.get(s) or self.retired[s])[0], 16)), 0.0
 
+    # the copy of a result to the host lands copy_s seconds after it is queued
+    copy_s = 0.0
+
+    def batch_download_async(self, s):
+        n = (self.slots.get(s) or self.retired[s])[0]
+        self.tickets = getattr(self, "tickets", {})
+        k = len(self.tickets)
+        self.tickets[k] = (self.clock() + self.copy_s, n)
+        self.log.append(("copy", n, self.clock()))
+        return k
+
+    def batch_download_collect(self, k, wait=True):
+        ready, n = self.tickets[k]
+        if self.clock() < ready:
+            if not wait:
+                return None
+            self.clock.t = ready
+        del self.tickets[k]
+        return np.zeros((n, 16)), 0.0
+
     def batch_autotune(self, probe_steps=2):
         return (32, 8, 2)
 
@@ -204,3 +224,23 @@ def test_split_lanes_route_by_dimensionality():
     _, _, made = mixed_run(True)
     cells = [sorted(n for e, n, _ in ctx.log if e == "add") for ctx in made]
     assert sorted(cells) == [[1000, 1000, 1000], [10 ** 6]]
+
+
+def test_retired_results_copy_beside_the_next_chunk():
+    """A retired request's result is copied to the host while the lane's next
+    chunk runs: the loop queues the copy, launches the chunk and collects the
+    copy by polling, so the request's completion time is its copy's landing,
+    not the end of the chunk, and nothing waits on the copy."""
+    reqs = requests([1000, 400000], [0.0, 0.0])
+    clock = SimClock()
+    ctx = ModelContext(clock, 1 << 40, {1000: 50, 400000: 10 ** 9})
+    ctx.copy_s = 1e-3
+    srv = serve.Server(check_every=50, max_steps=200, autotune=False, reserve_bytes=0,
+                       context=lambda: ctx, split_dims=False)
+    done, stats = srv.run(reqs, [None, None], clock=clock, sleep=clock.sleep)
+    small = next(d for d in done if d["cells"] == 1000)
+    t_copy = next(t for kind, n, t in ctx.log if kind == "copy" and n == 1000)
+    chunk = 50 * 400000 / ctx.rate  # the big request's next chunk alone
+    assert small["done"] >= t_copy + ctx.copy_s
+    assert small["done"] < t_copy + chunk  # collected before that chunk ended
+    assert stats["solves"] == 2
