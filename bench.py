@@ -246,6 +246,41 @@ def facelist_form(n, steps):
     return out
 
 
+def ena_form(nx, ny, n_v, cpu=True):
+    """ENA line-of-sight spectra (SET_ENA.cpp:544-780, Culc_ENA): the
+    reference's 178 angles (0.008 to pi - 0.008 rad by 1 degree) and n_v
+    speeds through the triangulated cell centres of an nx x ny half-plane
+    heliosphere.  Unit of work: one sample of one line of sight for one
+    speed (the survival and intensity update of SET_ENA.cpp:875-921).  The
+    CPU figure is the oracle (one thread, its brute-force triangle search
+    included) on one of the angles."""
+    import numpy as np
+
+    import oracle
+    from set_amd import ena, workloads
+    fm, W = workloads.heliosphere_face_mesh(nx, ny)
+    t0 = time.perf_counter()
+    tri = ena.triangulate(fm.cell_center)
+    t_tri = time.perf_counter() - t0
+    angles = np.arange(0.008, np.pi - 0.008, np.pi / 180.0)
+    ena.spectra(fm.cell_center, W, angles[:4], 64, tri=tri)  # warm-up
+    t0 = time.perf_counter()
+    j, n = ena.spectra(fm.cell_center, W, angles, n_v, tri=tri)
+    wall = time.perf_counter() - t0
+    work = float(n.sum()) * n_v
+    out = {"mesh": f"{nx}x{ny} half-plane heliosphere", "cells": fm.n_cell, "triangles": len(tri[0]),
+           "angles": len(angles), "speeds": n_v, "samples": int(n.sum()), "seconds": wall,
+           "value": work / wall, "unit": "sample-speed updates/s", "triangulation_s": t_tri}
+    if cpu:
+        k = len(angles) // 2
+        t0 = time.perf_counter()
+        ref, nref = oracle.ena_los(fm.cell_center, W, tri[0], angles[k], n_v)
+        tc = time.perf_counter() - t0
+        out["cpu_oracle_one_thread"] = {"value": nref * n_v / tc, "angle_rad": float(angles[k]), "seconds": tc}
+        out["max_rel_diff_vs_oracle"] = float(np.abs(j[k] - ref).max() / max(ref.max(), 1e-300))
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -292,7 +327,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--forms", default="fp64,mixed,heun,batch64,serving,dropin,facelist")
+    ap.add_argument("--forms", default="fp64,mixed,heun,batch64,serving,dropin,facelist,ena")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spin-s", type=float, default=0.3,
                     help="untimed device spin-up (seconds of FP64 steps) after the warm-up steps")
@@ -305,6 +340,8 @@ def main():
     ap.add_argument("--serve-max-steps", type=int, default=5000)
     ap.add_argument("--dropin-mesh", default="2000x1000")
     ap.add_argument("--dropin-steps", type=int, default=20)
+    ap.add_argument("--ena-mesh", default="400x200")
+    ap.add_argument("--ena-speeds", type=int, default=1000)
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -354,7 +391,9 @@ def main():
                      ("serving_fp64", lambda: serving_form(args.serve_requests, args.serve_max_steps)),
                      ("dropin_reference", lambda: dropin_form(*map(int, args.dropin_mesh.split("x")),
                                                               args.dropin_steps)),
-                     ("facelist_headline", lambda: facelist_form(n, args.dropin_steps))):
+                     ("facelist_headline", lambda: facelist_form(n, args.dropin_steps)),
+                     ("ena_los", lambda: ena_form(*map(int, args.ena_mesh.split("x")), args.ena_speeds,
+                                                  cpu=not args.no_cpu))):
         if name.split("_")[0] in wanted:
             try:
                 forms[name] = fn()

@@ -305,6 +305,24 @@ int setmhd_step_noalloc(const setmhd_config *cfg, int32_t nx, int32_t ny, int32_
                         const double *box /*[6]*/, const double *d_in, double *d_out, double dt,
                         void *stream, double *dt_next, double *residual);
 
+/* ENA line-of-sight spectra of a solution (SET_ENA/SET_ENA.cpp:544-956,
+ * Culc_ENA / Culc_ENA_the): for each angle (radians from the x-axis, in
+ * the x-y plane of the axisymmetric solution) the line of sight from 5 AU
+ * outwards, sampled every 1/30 AU until it leaves the triangulation, and
+ * for each of n_v particle speeds v_i = vL + (i + 1/2)(vR - vL)/n_v
+ * (vL = 1 km/s, vR = 1000 km/s) the ENA intensity jena[a*n_v + i] (before
+ * the reference's output factor 6.75186e7); samples[a] = samples taken.
+ * cell_center (3 per cell) and cell_par (Cell_par, 16 per cell) as in
+ * setmhd_upload_mesh; tri (3 per triangle, 0-based cell numbers, counter-
+ * clockwise) a triangulation of the cell centres -- the reference builds a
+ * CGAL Delaunay triangulation (:242-257) -- and tri_nb[3 t + k] the triangle
+ * across the edge opposite vertex k (-1 on the hull).  Synchronous on
+ * `device`; allocates its own device memory.  SETMHD_ERR_ARG also when a
+ * walk between triangles does not end (inconsistent neighbours). */
+int setmhd_ena_spectra(int32_t device, int32_t n_cell, const double *cell_center, const double *cell_par,
+                       int32_t n_tri, const int32_t *tri, const int32_t *tri_nb, int32_t n_angle,
+                       const double *angle, int32_t n_v, double *jena, int32_t *samples);
+
 /* Message of the last failure of this context ("" if none). */
 int setmhd_last_error(const setmhd_ctx *ctx, char *buf, size_t n);
 

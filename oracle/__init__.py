@@ -35,6 +35,10 @@ class Mesh(ctypes.Structure):
                 ("gran_neighbour", PI), ("gran_neighbour_tvd", PI), ("gran_type", PI)]
 
 
+class EnaMesh(ctypes.Structure):
+    _fields_ = [("n_cell", I32), ("n_tri", I32), ("cell_center", PD), ("cell_par", PD), ("tri", PI)]
+
+
 def reference_config():
     """The reference's production semantics: HLLC + LAX wedge, minmod TVD,
     Heun RK2, axisymmetric + charge-exchange sources, w = B = 0."""
@@ -76,6 +80,16 @@ def lib():
         L.orc_dt_scan.restype = D
         L.orc_run.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(Mesh), PD, PD, PD, PD, PD,
                               ctypes.c_int, ctypes.c_int]
+        L.orc_ena_velocity1.argtypes = [D, D]
+        L.orc_ena_velocity1.restype = D
+        L.orc_ena_maxwell.argtypes = [D] * 8
+        L.orc_ena_maxwell.restype = D
+        L.orc_ena_f.argtypes = [ctypes.c_int, D]
+        L.orc_ena_f.restype = D
+        L.orc_ena_int1.argtypes = [D, D]
+        L.orc_ena_int1.restype = D
+        L.orc_ena_los.argtypes = [ctypes.POINTER(EnaMesh), D, ctypes.c_int, PD]
+        L.orc_ena_los.restype = ctypes.c_int
         for f in ("orc_face_pass", "orc_cell_pass", "orc_run"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
@@ -170,3 +184,32 @@ class Solver:
         return lib().orc_cell_pass(ctypes.byref(self.cfg), ctypes.byref(self.mesh), int(stage),
                                    float(dt), _pd(self.W), _pd(self.W2), _pd(self.potok),
                                    self.nthreads)
+
+
+def ena_velocity1(u, cp):
+    return lib().orc_ena_velocity1(float(u), float(cp))
+
+
+def ena_maxwell(*a):
+    return lib().orc_ena_maxwell(*map(float, a))
+
+
+def ena_f(which, x):
+    return lib().orc_ena_f(int(which), float(x))
+
+
+def ena_int1(x, cp):
+    return lib().orc_ena_int1(float(x), float(cp))
+
+
+def ena_los(cell_center, cell_par, tri, the, n_v):
+    """(jENA per velocity, samples inside the mesh) of one line of sight at
+    angle `the` (radians from the x-axis) through the triangulated cell
+    centres (SET_ENA.cpp:782-931)."""
+    cc = np.ascontiguousarray(cell_center, np.float64)
+    cp = np.ascontiguousarray(cell_par, np.float64)
+    tr = np.ascontiguousarray(tri, np.int32)
+    m = EnaMesh(len(cc), len(tr), _pd(cc), _pd(cp), _pi(tr))
+    out = np.zeros(int(n_v))
+    n = lib().orc_ena_los(ctypes.byref(m), float(the), int(n_v), _pd(out))
+    return out, n

@@ -112,6 +112,22 @@ int orc_run(const orc_config *cfg, const orc_mesh *m, double *W, double *W2,
             double *potok, double *t, double *dt_next, int nsteps,
             int nthreads);
 
+/* ENA line of sight (SET_ENA/SET_ENA.cpp:341-956; ena.c).  A triangulation
+ * of the cell centres: tri[3*t..3*t+2] are 0-based cell numbers.           */
+typedef struct orc_ena_mesh {
+  int n_cell, n_tri;
+  const double *cell_center; /* [3*n_cell] */
+  const double *cell_par;    /* [16*n_cell] */
+  const int *tri;            /* [3*n_tri]  */
+} orc_ena_mesh;
+
+double orc_ena_velocity1(double u, double cp);
+double orc_ena_maxwell(double rho, double cp, double u, double v, double w, double vx, double vy,
+                       double vz);
+double orc_ena_f(int which, double x);
+double orc_ena_int1(double x, double cp);
+int orc_ena_los(const orc_ena_mesh *m, double the, int n_v, double *jena);
+
 #ifdef __cplusplus
 }
 #endif

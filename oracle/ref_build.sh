@@ -37,3 +37,17 @@ FFLAGS="-O2 -fPIC -ffp-contract=off"
 "$FC" -shared -fPIC -o libref_chlld.so storage.o ref_chlld_mod.o ref_bind.o
 rm -f ref_chlld_mod.f90 storage.mod ref_chlld_mod.mod ref_bind.mod storage.o ref_chlld_mod.o ref_bind.o
 echo "built $OUT/libref_chlld.so"
+
+# SET_ENA's rate functions (SET_ENA/SET_ENA.cpp): the file as a whole needs
+# CGAL, Eigen and Boost, none of which is here; these routines need only
+# <cmath>.  Cut, unmodified: the macros (:41-66), const_pi (:87),
+# Velosity_1 and maxwell_distribution (:341-370), int_1_f1/f2/f3 and int_1
+# (:426-542).
+ENA="${SET_REFERENCE:-/root/reference}/SET_ENA/SET_ENA.cpp"
+if [ -f "$ENA" ]; then
+  { sed -n '41,66p' "$ENA"; sed -n '87p' "$ENA"; sed -n '341,370p' "$ENA"; sed -n '426,542p' "$ENA"; } \
+    > "$OUT/ref_ena_cut.inc"
+  g++ -O2 -fPIC -ffp-contract=off -shared -I "$OUT" -o "$OUT/libref_ena.so" "$HERE/ref_ena_bind.cpp"
+  rm -f "$OUT/ref_ena_cut.inc"
+  echo "built $OUT/libref_ena.so"
+fi

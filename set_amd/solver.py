@@ -43,7 +43,7 @@ EXPORTS = (
     "setmhd_batch_poll", "setmhd_batch_wait", "setmhd_batch_query",
     "setmhd_batch_download", "setmhd_batch_autotune", "setmhd_batch_set_launch", "setmhd_mem_info",
     "setmhd_mesh_order", "setmhd_step_noalloc", "setmhd_batch_download_async",
-    "setmhd_batch_download_collect",
+    "setmhd_batch_download_collect", "setmhd_ena_spectra",
 )
 
 
@@ -112,6 +112,7 @@ def load():
                                         ctypes.POINTER(I32)]
     L.setmhd_mem_info.argtypes = [P, I64P, I64P]
     L.setmhd_mesh_order.argtypes = [I32, I32, PD, PD, PI, PI, PI]
+    L.setmhd_ena_spectra.argtypes = [I32, I32, PD, PD, I32, PI, PI, I32, PD, I32, PD, PI]
     L.setmhd_step_noalloc.argtypes = [ctypes.POINTER(Config), I32, I32, I32, PD, ctypes.c_void_p,
                                       ctypes.c_void_p, D, ctypes.c_void_p, PD, PD]
     for name in EXPORTS:
