@@ -17,12 +17,20 @@ def case(msg):
     print(f"[case] {msg}", flush=True)
 
 
-def test_spectra_match_oracle():
+@pytest.mark.parametrize("state", ["heliosphere", "warm"])
+def test_spectra_match_oracle(state):
+    """The heliosphere set-up (its dense, cold wind sphere absorbs almost every
+    ENA: intensities near 1e-70) and a warm variant (density capped at 1,
+    thermal speed ~30 km/s: intensities of order 1e-3)."""
     fm, W = workloads.heliosphere_face_mesh(160, 80)
+    if state == "warm":
+        W = W.copy()
+        W[:, 0] = np.minimum(W[:, 0], 1.0)
+        W[:, 4] = 10.0 * W[:, 0]
     tri, nb = ena.triangulate(fm.cell_center)
     angles = np.deg2rad([40.0, 65.0, 90.0, 115.0, 140.0])
     n_v = 96
-    case(f"ENA spectra 160x80 heliosphere, {len(tri)} triangles, {len(angles)} angles, {n_v} speeds")
+    case(f"ENA spectra 160x80 {state}, {len(tri)} triangles, {len(angles)} angles, {n_v} speeds")
     j, n = ena.spectra(fm.cell_center, W, angles, n_v, tri=(tri, nb))
     for a, th in enumerate(angles):
         ref, nref = oracle.ena_los(fm.cell_center, W, tri, th, n_v)
