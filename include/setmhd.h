@@ -240,9 +240,18 @@ int setmhd_batch_download_async(setmhd_ctx *ctx, int32_t slot, int32_t *ticket);
 
 /* Collects a queued copy.  wait = 0: returns at once, *done = 0 while the
  * copy runs; otherwise waits for it.  When done (*done = 1) cell_par and *t
- * (may be NULL) hold the slot's state and time and the ticket is free. */
+ * (either may be NULL: the copy is then dropped, e.g. after
+ * setmhd_batch_download_view) hold the slot's state and time and the
+ * ticket is free. */
 int setmhd_batch_download_collect(setmhd_ctx *ctx, int32_t ticket, int32_t wait, double *cell_par,
                                   double *t, int32_t *done);
+
+/* The landed copy in place, without a host copy: never blocks; when the
+ * copy has landed *done = 1 and *data points at its Cell_par (16 doubles
+ * per cell, pinned host memory of the library) until the ticket is
+ * collected; else *done = 0. */
+int setmhd_batch_download_view(setmhd_ctx *ctx, int32_t ticket, const double **data, double *t,
+                               int32_t *done);
 
 /* Times probe_steps steps of the admitted batch for each launch shape
  * (persistent grids of 1, 2, 4, 8, 64 blocks per CU, or one block per tile,

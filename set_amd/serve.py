@@ -139,13 +139,16 @@ class Server:
         for item in list(lane.downloads):
             ticket, rec = item
             ta = clock()
-            r = lane.ctx.batch_download_collect(ticket, wait=False)
-            spent["retire_collect"] += clock() - ta
+            r = lane.ctx.batch_download_view(ticket)  # the landed result, read in place
             if r is None:
+                spent["retire_collect"] += clock() - ta
                 continue
-            cp, t = r
+            view, t = r
+            cp = np.array(view) if keep else None
+            lane.ctx.batch_download_release(ticket)
+            spent["retire_collect"] += clock() - ta
             lane.downloads.remove(item)
-            done.append(dict(rec, done=clock() - t0, t=t, result=cp if keep else None))
+            done.append(dict(rec, done=clock() - t0, t=t, result=cp))
             got = True
         return got
 

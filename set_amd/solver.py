@@ -43,7 +43,7 @@ EXPORTS = (
     "setmhd_batch_poll", "setmhd_batch_wait", "setmhd_batch_query",
     "setmhd_batch_download", "setmhd_batch_autotune", "setmhd_batch_set_launch", "setmhd_mem_info",
     "setmhd_mesh_order", "setmhd_step_noalloc", "setmhd_batch_download_async",
-    "setmhd_batch_download_collect", "setmhd_ena_spectra",
+    "setmhd_batch_download_collect", "setmhd_batch_download_view", "setmhd_ena_spectra",
 )
 
 
@@ -108,6 +108,7 @@ def load():
     L.setmhd_batch_download.argtypes = [P, I32, PD, PD]
     L.setmhd_batch_download_async.argtypes = [P, I32, ctypes.POINTER(I32)]
     L.setmhd_batch_download_collect.argtypes = [P, I32, I32, PD, PD, ctypes.POINTER(I32)]
+    L.setmhd_batch_download_view.argtypes = [P, I32, ctypes.POINTER(PD), PD, ctypes.POINTER(I32)]
     L.setmhd_batch_autotune.argtypes = [P, I32, ctypes.POINTER(I32), ctypes.POINTER(I32),
                                         ctypes.POINTER(I32)]
     L.setmhd_mem_info.argtypes = [P, I64P, I64P]
@@ -368,6 +369,25 @@ class Context:
             return None
         del self._tickets[ticket]
         return cp, t.value
+
+    def batch_download_view(self, ticket):
+        """(Cell_par view, t) of a landed copy, reading the library's pinned
+        buffer in place (valid until batch_download_release), or None while
+        the copy runs; never blocks."""
+        ptr, t, done = PD(), D(), I32()
+        self._check(self._L.setmhd_batch_download_view(self._h, int(ticket), ctypes.byref(ptr),
+                                                       ctypes.byref(t), ctypes.byref(done)))
+        if not done.value:
+            return None
+        n = self._tickets[ticket]
+        return np.ctypeslib.as_array(ptr, shape=(n, 16)), t.value
+
+    def batch_download_release(self, ticket):
+        """Frees a landed copy's ticket (its buffer goes back to the pool)."""
+        done = I32()
+        self._check(self._L.setmhd_batch_download_collect(self._h, int(ticket), 1, None, None,
+                                                          ctypes.byref(done)))
+        del self._tickets[ticket]
 
     def batch_set_launch(self, tile=(32, 8), blocks_per_cu=0, dim=0):
         """Fixes the batch launch of 2-D (dim 2), 3-D (dim 3) or all (0)

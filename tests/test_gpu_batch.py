@@ -197,3 +197,32 @@ def test_dt_from_state_after_a_large_dt():
         cp, _ = c.download()
     ref, _ = alone(g, W, 3)
     np.testing.assert_array_equal(cp, ref)
+
+
+def test_async_download_view_and_collect():
+    """setmhd_batch_download_async of retired slots while the next step runs:
+    the copy read in place (download_view) and the copy collected into the
+    caller's array both equal the synchronous download."""
+    case("batch download_async: view and collect beside a running step")
+    gs = grids()
+    with solver.Context(solver.cartesian_config()) as c:
+        slots = [c.batch_add(g, W) for g, W in gs[:3]]
+        c.batch_step(6)
+        ref = [c.batch_download(s) for s in slots[:2]]
+        c.batch_remove(slots[0])
+        c.batch_remove(slots[1])
+        c.batch_step_async(5)  # slot 2 runs; the two retired copies go beside it
+        k0 = c.batch_download_async(slots[0])
+        k1 = c.batch_download_async(slots[1])
+        got1 = c.batch_download_collect(k1, wait=True)
+        r = None
+        while r is None:
+            r = c.batch_download_view(k0)
+        view0 = np.array(r[0])
+        t0 = r[1]
+        c.batch_download_release(k0)
+        c.batch_wait()
+    np.testing.assert_array_equal(view0, ref[0][0])
+    assert t0 == ref[0][1]
+    np.testing.assert_array_equal(got1[0], ref[1][0])
+    assert got1[1] == ref[1][1]

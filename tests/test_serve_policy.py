@@ -93,6 +93,13 @@ class ModelContext:
         self.log.append(("copy", n, self.clock()))
         return k
 
+    def batch_download_view(self, k):
+        ready, n = self.tickets[k]
+        return None if self.clock() < ready else (np.zeros((n, 16)), 0.0)
+
+    def batch_download_release(self, k):
+        del self.tickets[k]
+
     def batch_download_collect(self, k, wait=True):
         ready, n = self.tickets[k]
         if self.clock() < ready:
