@@ -248,7 +248,7 @@ def facelist_form(n, steps):
 
 def ena_form(nx, ny, n_v, cpu=True):
     """ENA line-of-sight spectra (SET_ENA.cpp:544-780, Culc_ENA): the
-    reference's 178 angles (0.008 to pi - 0.008 rad by 1 degree) and n_v
+    reference's 180 angles (0.008 to pi - 0.008 rad by 1 degree) and n_v
     speeds through the triangulated cell centres of an nx x ny half-plane
     heliosphere.  Unit of work: one sample of one line of sight for one
     speed (the survival and intensity update of SET_ENA.cpp:875-921).  The
@@ -262,7 +262,7 @@ def ena_form(nx, ny, n_v, cpu=True):
     t0 = time.perf_counter()
     tri = ena.triangulate(fm.cell_center)
     t_tri = time.perf_counter() - t0
-    angles = np.arange(0.008, np.pi - 0.008, np.pi / 180.0)
+    angles = ena.map_angles()
     ena.spectra(fm.cell_center, W, angles[:4], 64, tri=tri)  # warm-up
     t0 = time.perf_counter()
     j, n = ena.spectra(fm.cell_center, W, angles, n_v, tri=tri)

@@ -10,7 +10,11 @@ triangulation wherever no four centres are cocircular.  `spectra` runs
 setmhd_ena_spectra (libsetmhd.so); the text writers follow the reference's
 output files: the spectrum of one line of sight (Culc_ENA_the,
 SET_ENA.cpp:934-952) and the intensities at fixed energies per angle
-(Culc_ENA, :700-779).
+(Culc_ENA, :700-779), with the numbers at the stream's default precision
+(6 significant digits, printf's %g) as the reference writes them.  `run`
+and `run_map` are the reference's main (:958-975): the FCMHD input file for
+the cell centres, the solver's output file for the state, then one line of
+sight or the angle map.
 """
 import ctypes
 
@@ -43,6 +47,17 @@ def triangulate(cell_center):
     return np.ascontiguousarray(tri), np.ascontiguousarray(nb)
 
 
+def map_angles():
+    """The angles of Culc_ENA's map (:549): 0.008 rad, then repeatedly plus
+    pi/180 while below pi - 0.008, accumulated as the reference's loop
+    accumulates them -- 180 angles."""
+    out, th = [], 0.008
+    while th < np.pi - 0.008:
+        out.append(th)
+        th = th + np.pi / 180.0
+    return np.array(out)
+
+
 def speeds(n_v):
     """The particle speeds of a spectrum: v_i = vL + (i + 1/2)(vR - vL)/n_v,
     vL = 1 km/s, vR = 1000 km/s (:815-836)."""
@@ -52,7 +67,8 @@ def speeds(n_v):
 
 def energies_ev(n_v):
     """velocity_to_energy of the speeds (:408-412), eV."""
-    return (speeds(n_v) / KMS) ** 2 * 5.226 * 0.001
+    x = speeds(n_v) / KMS
+    return x * x * 5.226 * 0.001
 
 
 def spectra(cell_center, cell_par, angles, n_v=1000, tri=None, device=0):
@@ -102,18 +118,57 @@ def angle_map(jena, n_v, fix_high_energies=False):
     return out
 
 
-def write_spectrum(path, jena_row):
+def _num(x, exact):
+    return repr(float(x)) if exact else "%g" % x
+
+
+def write_spectrum(path, jena_row, exact=False):
     """The spectrum file of one line of sight (Culc_ENA_the, :934-952):
-    energy (keV), speed (km/s), intensity per line."""
+    energy (keV), speed (km/s), intensity per line.  exact=True writes every
+    digit of each double instead of the reference's 6."""
     v = speeds(len(jena_row))
     with open(path, "w") as f:
         for vi, ji in zip(v, jena_row):
-            f.write(f"{(vi / KMS) ** 2 * 5.226 * 0.000001} {vi / KMS} {ji * FLUX}\n")
+            e = (vi / KMS) * (vi / KMS) * 5.226 * 0.000001
+            f.write(f"{_num(e, exact)} {_num(vi / KMS, exact)} {_num(ji * FLUX, exact)}\n")
 
 
-def write_angle_map(path, angles, amap):
+def write_angle_map(path, angles, amap, exact=False):
     """Culc_ENA's jENA_angle_test.txt layout (:773-776): the angle, then the
     14 intensities."""
     with open(path, "w") as f:
         for th, row in zip(angles, amap):
-            f.write(" ".join([repr(float(th))] + [repr(float(x)) for x in row]) + "\n")
+            f.write(" ".join([_num(th, exact)] + [_num(x, exact) for x in row]) + "\n")
+
+
+def read_solution(mesh_file, state_file):
+    """Set_Storage then ReadUpdatedStorage (:115-240): the cell centres from
+    the FCMHD input file and Cell_par from the solver's output file (time,
+    16 x N_cell, the 321.0 sentinel).  Returns (cell_center, cell_par, time)."""
+    from . import fcmhd
+    _, fm, _, _ = fcmhd.read_input(mesh_file)
+    t, cp = fcmhd.fill_data(state_file, fm.n_cell)
+    return fm.cell_center, cp, t
+
+
+def run(mesh_file, state_file, the, name, n_v=1000, out_dir=".", device=0):
+    """main + Culc_ENA_the (:958-975, :782-956): the spectrum along the line
+    of sight at angle `the` (rad), written to <out_dir>/jENA_<name>.txt.
+    Returns (jENA row, samples)."""
+    import os
+    cc, cp, _ = read_solution(mesh_file, state_file)
+    j, n = spectra(cc, cp, [the], n_v, device=device)
+    write_spectrum(os.path.join(out_dir, f"jENA_{name}.txt"), j[0])
+    return j[0], int(n[0])
+
+
+def run_map(mesh_file, state_file, n_v=1000, out_dir=".", fix_high_energies=False, device=0):
+    """main + Culc_ENA (:544-780): the map over map_angles(), written to
+    <out_dir>/jENA_angle_test.txt.  Returns (angles, map)."""
+    import os
+    cc, cp, _ = read_solution(mesh_file, state_file)
+    th = map_angles()
+    j, _ = spectra(cc, cp, th, n_v, device=device)
+    amap = angle_map(j, n_v, fix_high_energies)
+    write_angle_map(os.path.join(out_dir, "jENA_angle_test.txt"), th, amap)
+    return th, amap

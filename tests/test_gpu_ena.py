@@ -73,3 +73,41 @@ def test_inconsistent_neighbours_rejected():
     with pytest.raises(solver.SetMHDError) as e:
         ena.spectra(cc, W, [np.deg2rad(30.0)], 8, tri=(tri, bad))
     assert e.value.status == solver.ERR_ARG
+
+
+def test_file_driver_matches_oracle(tmp_path):
+    """The reference's main from files (SET_ENA.cpp:958-975): the FCMHD input
+    file for the centres and a solver output file for the state; the written
+    spectrum (6 significant digits) and the angle map equal the oracle's
+    numbers written the same way."""
+    from set_amd import fcmhd
+    fm, W = workloads.heliosphere_face_mesh(80, 40)
+    W = W.copy()
+    W[:, 0] = np.minimum(W[:, 0], 1.0)
+    W[:, 4] = 10.0 * W[:, 0]
+    fcmhd.write_input(tmp_path / "FCMHD_1.bin", fm, W)
+    fcmhd.save_storage(tmp_path / "FCMHD_1.8_out.bin", 0.5, W)
+    # the reference's main looks along 3 degrees on its fine mesh; on this
+    # coarse one the start point 5 AU out at 3 degrees lies below the lowest
+    # row of centres, outside the triangulation, so the line is taken at 60
+    the = 60.0 * np.pi / 180.0
+    case("ENA file driver, 80x40, 60 degrees, 200 speeds")
+    j, n = ena.run(tmp_path / "FCMHD_1.bin", tmp_path / "FCMHD_1.8_out.bin", the, "1.8(60)", n_v=200,
+                   out_dir=tmp_path)
+    tri, _ = ena.triangulate(fm.cell_center)
+    ref, nref = oracle.ena_los(fm.cell_center, W, tri, the, 200)
+    assert n == nref > 100 and ref.max() > 0
+    got = np.loadtxt(tmp_path / "jENA_1.8(60).txt")
+    assert got.shape == (200, 3)
+    np.testing.assert_allclose(got[:, 2], ref * ena.FLUX, rtol=1e-5, atol=1e-10 * ref.max() * ena.FLUX)
+    case("ENA angle map from files, 80x40, 180 angles, 64 speeds")
+    th, amap = ena.run_map(tmp_path / "FCMHD_1.bin", tmp_path / "FCMHD_1.8_out.bin", n_v=64, out_dir=tmp_path)
+    rows = np.loadtxt(tmp_path / "jENA_angle_test.txt")
+    assert rows.shape == (180, 15)
+    np.testing.assert_allclose(rows[:, 0], th, rtol=1e-5)
+    assert np.all(amap[0] == 0.0)  # 0.008 rad: the start point lies outside this coarse mesh
+    for a in (60, 90, 120):
+        r, nr = oracle.ena_los(fm.cell_center, W, tri, th[a], 64)
+        assert nr > 100
+        want = ena.angle_map(r[None, :], 64)[0]
+        np.testing.assert_allclose(amap[a], want, rtol=0, atol=1e-10 * max(want.max(), 1e-300))
