@@ -58,7 +58,7 @@ program fcmhd_driver
   nargs = command_argument_count()
   if (nargs < 3) then
     print *, 'usage: fcmhd_driver input.bin output.bin all_step [print_step] [restart.bin|-] ', &
-             '[cartesian|reference] [probes.txt|-] [dump|-] [reorder]'
+             '[cartesian|reference] [probes.txt|-] [dump|-] [reorder|-] [continue]'
     stop 2
   end if
   call get_command_argument(1, fin)
