@@ -111,3 +111,43 @@ def test_file_driver_matches_oracle(tmp_path):
         assert nr > 100
         want = ena.angle_map(r[None, :], 64)[0]
         np.testing.assert_allclose(amap[a], want, rtol=0, atol=1e-10 * max(want.max(), 1e-300))
+
+
+@pytest.mark.parametrize("n_v", [300, 600])
+def test_wide_speed_groups_match_oracle(n_v):
+    """Each lane carries 4 (256 <= n_v < 512) or 8 (n_v >= 512) speeds, the
+    last group partly empty."""
+    fm, W = workloads.heliosphere_face_mesh(80, 40)
+    W = W.copy()
+    W[:, 0] = np.minimum(W[:, 0], 1.0)
+    W[:, 4] = 10.0 * W[:, 0]
+    tri, nb = ena.triangulate(fm.cell_center)
+    angles = np.deg2rad([70.0, 110.0])
+    case(f"ENA spectra 80x40 warm, {n_v} speeds")
+    j, n = ena.spectra(fm.cell_center, W, angles, n_v, tri=(tri, nb))
+    for a, th in enumerate(angles):
+        ref, nref = oracle.ena_los(fm.cell_center, W, tri, th, n_v)
+        assert n[a] == nref > 1000 and ref.max() > 0
+        np.testing.assert_allclose(j[a], ref, rtol=0, atol=1e-10 * ref.max())
+
+
+def test_short_lines_one_segment():
+    """A mesh a few AU across: every line of sight holds fewer samples than
+    one segment, some none at all."""
+    g = mesh.Grid(40, 20, 1, -0.03, 0.03, 0.0, 0.03)
+    cc = g.cell_centers()
+    rng = np.random.default_rng(11)
+    W = np.zeros((len(cc), 16))
+    W[:, [0, 1, 2, 4]] = [0.2, -1.0, 0.3, 2.0]
+    W[:, 0] *= rng.uniform(0.8, 1.2, len(cc))
+    W[:, 8:12] = [0.1, -2.0, 0.1, 0.4]
+    W[:, 12:16] = [0.05, -1.4, 0.1, 0.8]
+    tri, nb = ena.triangulate(cc)
+    angles = np.deg2rad([20.0, 60.0, 90.0, 150.0, 179.0])
+    case("ENA spectra on a 0.06 x 0.03 mesh, 5 angles, 64 speeds")
+    j, n = ena.spectra(cc, W, angles, 64, tri=(tri, nb))
+    assert n.max() < 256 and n.max() > 50
+    for a, th in enumerate(angles):
+        ref, nref = oracle.ena_los(cc, W, tri, th, 64)
+        assert n[a] == nref
+        np.testing.assert_allclose(j[a], ref, rtol=0, atol=1e-10 * max(ref.max(), 1e-300))
