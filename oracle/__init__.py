@@ -130,6 +130,15 @@ def source_mk(cell_par16):
     return out
 
 
+def default_threads():
+    """OMP_NUM_THREADS when set, else the host's cores, at most 16."""
+    try:
+        n = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        n = 0
+    return n if n > 0 else max(1, min(16, os.cpu_count() or 1))
+
+
 class Solver:
     """Holds a FaceMesh and the state arrays (Cell_par, Cell_par2,
     Gran_POTOK) and runs the oracle's step loop on them."""
@@ -151,7 +160,7 @@ class Solver:
         self.potok = np.zeros((fm.n_gran, 9))
         self.t = 0.0
         self.dt_next = 2.0e-9  # cuf_kernel.cuf:758
-        self.nthreads = nthreads or os.cpu_count() or 1
+        self.nthreads = nthreads or default_threads()
 
     def dt_scan(self):
         return lib().orc_dt_scan(ctypes.byref(self.cfg), ctypes.byref(self.mesh), _pd(self.W),

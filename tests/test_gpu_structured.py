@@ -122,6 +122,31 @@ def test_3d_parity_within_conditioning():
     assert np.all(e[:5] <= 1e-10), e
 
 
+def test_3d_parity_500_steps():
+    """The north-star bound in 3-D: relative L1 <= 1e-9 after 500 steps for
+    rho, m and E (their 1-ulp input sensitivity after 500 steps is ~1e-12);
+    the field components, whose 1-ulp sensitivity reaches ~5e-4 by then, are
+    held to 10x the larger of the oracle's sensitivities to a one-ulp step of
+    its input up and down."""
+    case("structured 3-D 32x28x24 HLLD Euler FP64 steps=500 vs oracle and 1-ulp sensitivity")
+    g = mesh.grid_3d(32, 28, 24)
+    W = mesh.heliosphere_state(g, wind=mesh.Wind(r_in=0.3))
+    s, _ = oracle_run(g, W, 500)
+    sens = np.zeros(8)
+    for d in (np.inf, -np.inf):
+        Wp = W.copy()
+        Wp[:, :8] = np.nextafter(Wp[:, :8], d)
+        sp, _ = oracle_run(g, Wp, 500)
+        sens = np.maximum(sens, rel_err(cons(sp.W), cons(s.W), "l1"))
+    cp, t, _ = gpu_run(g, W, 500)
+    assert t == pytest.approx(s.t, rel=1e-12)
+    e = rel_err(cons(cp), cons(s.W), "l1")
+    print("rel L1 per var", e, "1-ulp sensitivity", sens)
+    assert np.all(sens[:5] < 1e-10), sens
+    assert np.all(e[:5] <= 1e-9), e
+    assert np.all(e[5:] <= 10.0 * sens[5:]), (e, sens)
+
+
 def test_heun_parity():
     case("structured 2-D 200x100 Heun RK2 FP64 steps=40")
     g = mesh.grid_2d(200, 100)
