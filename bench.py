@@ -424,6 +424,16 @@ def main():
         fp["compute"] = {"flops_per_cell_update": fl, "achieved": tf, "peak": PEAK_FP64_TFLOPS,
                          "unit": "TFLOP/s (FP64 vector)", "frac": tf / PEAK_FP64_TFLOPS,
                          "flops_source": f"profiles/fused_fp64_euler_{n}cube.txt (VALU counters)"}
+        # the VALU-issue ceiling that binds this kernel before HBM does:
+        # 1024 SIMDs x 2.4 GHz / 4 cycles per wave instruction over the
+        # kernel's wave instructions per cell-update (VALU counter / 64)
+        vi = from_profiles(f"fused_fp64_euler_{n}cube", "valu_instructions_per_cell_update")
+        if vi:
+            ceil = 1024 * 2.4e9 / 4 / (vi / 64.0)
+            fp["compute"]["valu_issue_ceiling"] = {
+                "valu_instructions_per_cell_update": vi, "cell_updates_per_s": ceil,
+                "hbm_frac_at_ceiling": ceil * 128 / (PEAK_HBM_GBS * 1e9),
+                "kernel_frac_of_ceiling": g.n_cell / (fp["kernel_ms_per_launch"] * 1e-3) / ceil}
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
