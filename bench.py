@@ -193,7 +193,8 @@ def dropin_form(nx, ny, steps):
     """The drop-in face-list path at SET's production semantics (HLLC with the
     LAX wedge, minmod TVD, Heun, axisymmetric and charge-exchange sources,
     w = B = 0, reference initial dt 2e-9) on a half-plane heliosphere mesh:
-    the exact build (bit for bit with the oracle) and the fast build."""
+    the exact build (bit for bit with the oracle) and the fast build, each
+    with two launches per Heun stage and with one (SETMHD_F_PER_CELL)."""
     import numpy as np
 
     from set_amd import mesh, solver, workloads
@@ -206,6 +207,8 @@ def dropin_form(nx, ny, steps):
     out = {"mesh": f"{nx}x{ny} half-plane heliosphere face list", "cells": fm.n_cell,
            "faces": fm.n_gran, "steps": steps, "time_scheme": "heun"}
     for name, m, w, flags in (("exact", fm, W, 0), ("fast", fm, W, solver.F_FAST),
+                              ("exact_per_cell", fm, W, solver.F_PER_CELL),
+                              ("fast_per_cell", fm, W, solver.F_FAST | solver.F_PER_CELL),
                               ("exact_random_numbering", fr, W[cp], 0),
                               ("exact_random_numbering_reorder", fr, W[cp], solver.F_REORDER)):
         cfg = solver.reference_config()

@@ -58,6 +58,12 @@ enum setmhd_status {
                                   of their centres and the faces by normal axis and first cell
                                   (setmhd_mesh_order); every call still takes and returns the
                                   caller's numbering.  Results are bit for bit the same. */
+#define SETMHD_F_PER_CELL 64   /* face-list meshes, Heun: one launch per stage, each cell forming
+                                  the faces of its Cell_gran slots itself (cuf_solvers.cuf:243-794
+                                  without the Gran_POTOK round trip of cuf_kernel.cuf:849-876);
+                                  every interior face is formed by both its cells.  Results are
+                                  bit for bit the same.  Ignored (two launches per stage) when a
+                                  face is missing from its first cell's Cell_gran. */
 
 #define SETMHD_RIEMANN_LAX 0
 #define SETMHD_RIEMANN_HLL 1

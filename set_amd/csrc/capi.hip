@@ -35,6 +35,7 @@ struct setmhd_ctx {
   int kind = 0;  // 0 none, 1 unstructured, 2 structured
   long long n_cell = 0, n_gran = 0;
   long long step = 0;  // steps done since upload (selects the dt slot)
+  bool per_cell_ok = false;  // every face is in its first cell's Cell_gran (SETMHD_F_PER_CELL)
   // device scalars: dt ring [3], time [1], residual [2]; error words: [0] the
   // steps', [1] the batch admissions' (their scans run beside a step)
   double* d_scal = nullptr;
@@ -432,6 +433,10 @@ template <class... A>
 hipError_t cell_pass(const setmhd_ctx* c, A... a) {
   return (c->cfg.flags & SETMHD_F_FAST) ? fast::launch_cell_pass(a...) : exact::launch_cell_pass(a...);
 }
+template <class... A>
+hipError_t cell_fused(const setmhd_ctx* c, A... a) {
+  return (c->cfg.flags & SETMHD_F_FAST) ? fast::launch_cell_fused(a...) : exact::launch_cell_fused(a...);
+}
 
 // The uniform box of nx x ny x nz cells spanning box[6] (x0,x1,y0,y1,z0,z1;
 // z ignored in 2-D), with the run's CFL number, dt cap and held radius.
@@ -570,6 +575,10 @@ int unstructured_step(setmhd_ctx* c) {
     if (e == hipSuccess)
       e = cell_pass(c, c->stream, c->um, k, 0, c->d_W, c->d_W2, c->d_H, c->d_potok, r, cur,
                            c->d_err);
+  } else if ((c->cfg.flags & SETMHD_F_PER_CELL) && c->per_cell_ok) {  // :847-877 in two launches
+    e = cell_fused(c, c->stream, c->um, k, 1, c->d_W, c->d_W2, c->d_H, c->d_potok, r, cur, 1, 1, c->d_err);
+    if (e == hipSuccess)
+      e = cell_fused(c, c->stream, c->um, k, 2, c->d_W, c->d_W2, c->d_H, c->d_potok, r, cur, 0, 0, c->d_err);
   } else {  // :847-877
     e = face_pass(c, c->stream, c->um, k, c->d_W, c->d_potok, r, cur, 1, 1, c->d_err);
     if (e == hipSuccess)
@@ -783,6 +792,15 @@ int setmhd_upload_mesh(setmhd_ctx* c, int32_t n_cell, int32_t n_gran, const doub
     gran_neighbour = r_nb.data();
     gran_neighbour_tvd = r_tvd.data();
     gran_type = r_ty.data();
+  }
+  // SETMHD_F_PER_CELL needs every face in its first cell's Cell_gran: that
+  // cell writes the face's Gran_POTOK row and its CFL candidate counts
+  c->per_cell_ok = true;
+  for (size_t f = 0; f < nf && c->per_cell_ok; f++) {
+    const int32_t* g = cell_gran + 6 * (size_t)(gran_neighbour[2 * f] - 1);
+    bool listed = false;
+    for (int k = 0; k < 6; k++) listed |= g[k] == (int32_t)f + 1;
+    c->per_cell_ok = listed;
   }
   // one block for the geometry (and the cell permutation)
   size_t bytes = nc * (3 + 1) * 8 + nf * (3 + 1 + 3) * 8 + nc * 6 * 4 + nf * (2 + 2 + 1) * 4 + 64 +

@@ -27,6 +27,8 @@ struct UCfg {
 namespace exact {
 hipError_t launch_face_pass(hipStream_t s, const UMesh& m, const UCfg& c, const double* S,
                             double* potok, DtRing dt, int cur, int reduce, int book, int* err);
+hipError_t launch_cell_fused(hipStream_t s, const UMesh& m, const UCfg& c, int stage, double* W, double* W2,
+                             const double* H, double* potok, DtRing dt, int cur, int reduce, int book, int* err);
 hipError_t launch_cell_pass(hipStream_t s, const UMesh& m, const UCfg& c, int stage, double* W,
                             double* W2, const double* H, const double* potok, DtRing dt, int cur,
                             int* err);
@@ -34,6 +36,8 @@ hipError_t launch_cell_pass(hipStream_t s, const UMesh& m, const UCfg& c, int st
 namespace fast {
 hipError_t launch_face_pass(hipStream_t s, const UMesh& m, const UCfg& c, const double* S,
                             double* potok, DtRing dt, int cur, int reduce, int book, int* err);
+hipError_t launch_cell_fused(hipStream_t s, const UMesh& m, const UCfg& c, int stage, double* W, double* W2,
+                             const double* H, double* potok, DtRing dt, int cur, int reduce, int book, int* err);
 hipError_t launch_cell_pass(hipStream_t s, const UMesh& m, const UCfg& c, int stage, double* W,
                             double* W2, const double* H, const double* potok, DtRing dt, int cur,
                             int* err);

@@ -13,7 +13,7 @@ module setmhd
   integer(c_int), parameter :: SETMHD_OK = 0
   integer(c_int), parameter :: SETMHD_F_ZERO_BW = 1, SETMHD_F_AXISYM = 2
   integer(c_int), parameter :: SETMHD_F_SOURCE_MK = 4, SETMHD_F_HELIO_HACKS = 8
-  integer(c_int), parameter :: SETMHD_F_FAST = 16, SETMHD_F_REORDER = 32
+  integer(c_int), parameter :: SETMHD_F_FAST = 16, SETMHD_F_REORDER = 32, SETMHD_F_PER_CELL = 64
 
   type, bind(C) :: setmhd_config
     integer(c_int32_t) :: riemann, recon, time_scheme, flags

@@ -28,7 +28,7 @@ PD = ctypes.POINTER(D)
 PI = ctypes.POINTER(I32)
 
 OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_RHO, ERR_REGION, ERR_FACE_RHO, ERR_UNSUPPORTED = range(8)
-F_ZERO_BW, F_AXISYM, F_SOURCE_MK, F_HELIO_HACKS, F_FAST, F_REORDER = 1, 2, 4, 8, 16, 32
+F_ZERO_BW, F_AXISYM, F_SOURCE_MK, F_HELIO_HACKS, F_FAST, F_REORDER, F_PER_CELL = 1, 2, 4, 8, 16, 32, 64
 RIEMANN_LAX, RIEMANN_HLL, RIEMANN_HLLC, RIEMANN_HLLD = range(4)
 TIME_EULER, TIME_HEUN = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1

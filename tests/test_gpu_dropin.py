@@ -420,3 +420,71 @@ def test_fortran_driver_restart(tmp_path, mode):
         assert np.all(err <= 1e-12), err
         assert t3 == pytest.approx(s.t, rel=1e-13)
         np.testing.assert_array_equal(cp3[:, 8:], cp1[:, 8:])
+
+
+def run_gpu(fm, W, cfg, nsteps, dt0, flags=0):
+    cfg.flags |= flags
+    with solver.Context(cfg) as c:
+        c.upload_mesh(fm, W)
+        c.set_dt(dt0)
+        c.step(nsteps)
+        cp, t = c.download()
+        _, dtn = c.sync()
+        pot = c.download_potok()
+    return cp, t, dtn, pot
+
+
+def assert_same_run(a, b):
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[3], b[3])
+    assert a[1] == b[1] and a[2] == b[2]
+
+
+@pytest.mark.parametrize("build", ["exact", "fast", "reorder"])
+def test_per_cell_heun_same_bits(build):
+    """SETMHD_F_PER_CELL (one launch per Heun stage, each cell forming its
+    own faces) gives the bits of the face pass + cell pass: state, Gran_POTOK,
+    time and next dt, at the reference's semantics (HLLC + LAX wedge, TVD,
+    axisymmetric and charge-exchange sources, held cells)."""
+    case(f"dropin per-cell Heun vs two launches per stage, {build}, heliosphere 120x60")
+    fm, W = helio_mesh(120, 60)
+    extra = {"exact": 0, "fast": solver.F_FAST, "reorder": solver.F_REORDER}[build]
+    ocfg = oracle.reference_config()
+    a = run_gpu(fm, W, gpu_cfg(ocfg), 12, 2.0e-9, extra)
+    b = run_gpu(fm, W, gpu_cfg(ocfg), 12, 2.0e-9, extra | solver.F_PER_CELL)
+    assert_same_run(a, b)
+
+
+def test_per_cell_heun_bitwise_against_oracle_3d():
+    """3-D HLLD with TVD and Heun on the per-cell path: bit for bit the
+    oracle's (no transcendental in this configuration)."""
+    case("dropin per-cell 3-D 12x10x8 HLLD TVD Heun vs oracle")
+    g = mesh.grid_3d(12, 10, 8)
+    fm = mesh.face_mesh(g)
+    W = mesh.heliosphere_state(g, wind=mesh.Wind(r_in=0.35, speed=3.0))
+    ocfg = oracle.cartesian_config(recon=1, time_scheme=1)
+    dt0 = oracle.Solver(fm, W, ocfg).dt_scan()
+    s = oracle.Solver(fm, W, ocfg)
+    s.dt_next = dt0
+    assert s.run(8) == 0
+    cp, t, dtn, pot = run_gpu(fm, W, gpu_cfg(ocfg), 8, dt0, solver.F_PER_CELL)
+    np.testing.assert_array_equal(cp[:, :8], s.W[:, :8])
+    np.testing.assert_array_equal(pot, s.potok)
+    assert t == s.t and dtn == s.dt_next
+
+
+def test_per_cell_falls_back_when_a_face_is_unlisted():
+    """A boundary face missing from its cell's Cell_gran: the face pass still
+    counts its CFL candidate, the per-cell form could not, so the flag falls
+    back to two launches per stage and the bits stay those of that form."""
+    case("dropin per-cell fallback, heliosphere 60x30 with an unlisted face")
+    fm, W = helio_mesh(60, 30)
+    f = int(np.flatnonzero(fm.gran_neighbour[:, 1] == -1)[5])
+    c1 = fm.gran_neighbour[f, 0] - 1
+    cg = fm.cell_gran.copy()
+    cg[c1][cg[c1] == f + 1] = 0
+    fm.cell_gran = cg
+    ocfg = oracle.reference_config()
+    a = run_gpu(fm, W, gpu_cfg(ocfg), 6, 2.0e-9)
+    b = run_gpu(fm, W, gpu_cfg(ocfg), 6, 2.0e-9, solver.F_PER_CELL)
+    assert_same_run(a, b)
