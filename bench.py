@@ -189,6 +189,33 @@ def serving_form(n_requests, max_steps, precision=0):
                            "rate_ratio": (work / b["wall_s"]) / (q["cell_updates"] / busy_q)}}
 
 
+def small_form(nx=400, ny=200, steps=500):
+    """The CPU-runnable 2-D case (400 x 200, the reference's own size class):
+    the structured step in FP64 and mixed and the drop-in face list (HLLD,
+    forward Euler), timed with HIP events over `steps` steps.  A step of so
+    small a grid is bound by the latency of its row march, not by HBM."""
+    from set_amd import mesh, solver
+    g = mesh.grid_2d(nx, ny)
+    W = mesh.heliosphere_state(g)
+    out = {"grid": f"{nx}x{ny}", "cells": g.n_cell, "steps": steps}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        with solver.Context(solver.cartesian_config(precision=prec)) as c:
+            c.upload_structured(g, W)
+            c.dt_from_state()
+            c.step(50)
+            c.sync()
+            ms = c.step_timed(steps)
+        out[name] = {"us_per_step": ms * 1e3 / steps, "value": g.n_cell * steps / (ms * 1e-3)}
+    with solver.Context(solver.cartesian_config()) as c:
+        c.upload_mesh(mesh.face_mesh(g), W)
+        c.dt_from_state()
+        c.step(50)
+        c.sync()
+        ms = c.step_timed(steps)
+    out["dropin_exact"] = {"us_per_step": ms * 1e3 / steps, "value": g.n_cell * steps / (ms * 1e-3)}
+    return out
+
+
 def dropin_form(nx, ny, steps):
     """The drop-in face-list path at SET's production semantics (HLLC with the
     LAX wedge, minmod TVD, Heun, axisymmetric and charge-exchange sources,
@@ -330,7 +357,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--forms", default="fp64,mixed,heun,batch64,serving,dropin,facelist,ena")
+    ap.add_argument("--forms", default="fp64,mixed,heun,batch64,serving,small,dropin,facelist,ena")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spin-s", type=float, default=0.3,
                     help="untimed device spin-up (seconds of FP64 steps) after the warm-up steps")
@@ -392,6 +419,7 @@ def main():
     for name, fn in (("batch64_fp64", lambda: batch_form(0, args.batch_steps, 20)),
                      ("batch64_mixed", lambda: batch_form(1, args.batch_steps, 20)),
                      ("serving_fp64", lambda: serving_form(args.serve_requests, args.serve_max_steps)),
+                     ("small_2d", lambda: small_form()),
                      ("dropin_reference", lambda: dropin_form(*map(int, args.dropin_mesh.split("x")),
                                                               args.dropin_steps)),
                      ("facelist_headline", lambda: facelist_form(n, args.dropin_steps)),
