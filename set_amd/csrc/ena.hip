@@ -91,7 +91,7 @@ __device__ __forceinline__ void fits(double x, double f[3]) {
 // thermal speed, and the speed-independent factors of int_1 (:538-542):
 // b = 1 - a_2 ln cp enters as b^2 and 2 a_2 b, the prefactor cp/sqrtpi^3.
 struct Pop {
-  double rho, u, v, cp, bb, b2a, cpf;
+  double rho, u, v, cp, icp, bb, b2a, cpf;
 };
 
 __device__ __forceinline__ Pop population(double rho, double u, double v, double cp) {
@@ -100,6 +100,7 @@ __device__ __forceinline__ Pop population(double rho, double u, double v, double
   p.u = u;
   p.v = v;
   p.cp = cp;
+  p.icp = 1.0 / cp;
   const double b = 1.0 - kA2 * log(cp);
   p.bb = b * b;
   p.b2a = 2.0 * kA2 * b;
@@ -110,10 +111,13 @@ __device__ __forceinline__ Pop population(double rho, double u, double v, double
 // charge-exchange rate of a particle at relative speed u in population P
 // (:879-910): beyond u/cp = 7 the mean relative speed Velosity_1
 // (:341-351) and the cross-section at it, below that the fitted Maxwellian
-// average int_1 (:538-542)
+// average int_1 (:538-542).  The divisions by cp and by Kn_ are
+// multiplications by reciprocals formed once (within an ulp of the
+// quotient; the spectra move by 1e-15 of their peaks).
+constexpr double kInvKn = 1.0 / kKn;
 __device__ __forceinline__ double rate(double u, const Pop& P) {
   const double cp = P.cp;
-  const double x = u / cp;
+  const double x = u * P.icp;
   if (x > 7.0) {
     double uz;
     if (u < 0.00001)
@@ -124,12 +128,12 @@ __device__ __forceinline__ double rate(double u, const Pop& P) {
           // library both return 1 from x = 6 on), so this is the same double
       uz = u + cp * cp / (2.0 * u);
     const double s = 1.0 - kA2 * log(uz);
-    return P.rho * uz * (s * s) / kKn;
+    return P.rho * uz * (s * s) * kInvKn;
   }
   double f[3];
   fits(x, f);
   const double i1 = P.cpf * (P.bb * f[0] - P.b2a * f[1] + (kA2 * kA2) * f[2]);
-  return P.rho * i1 / kKn;
+  return P.rho * i1 * kInvKn;
 }
 
 struct Args {
@@ -320,13 +324,14 @@ __global__ __launch_bounds__(256) void k_ena_march(Args a) {
   x0 = a.seg_xy[2 * g];
   y0 = a.seg_xy[2 * g + 1];
   const double vL = 1 * kKms, vR = 1000 * kKms;
-  double v[SPL], S[SPL], j[SPL];
+  double v[SPL], dsv[SPL], S[SPL], j[SPL];
   bool mine[SPL];
 #pragma unroll
   for (int q = 0; q < SPL; q++) {
     const int i = (chunk * SPL + q) * 64 + lane;
     mine[q] = i < a.n_v;
     v[q] = vL + (i + 0.5) * ((vR - vL) / a.n_v);
+    dsv[q] = ds / v[q];
     S[q] = 1.0;
     j[q] = 0.0;
   }
@@ -350,7 +355,7 @@ __global__ __launch_bounds__(256) void k_ena_march(Args a) {
     const double r = sqrt(x0 * x0 + y0 * y0);
     const double nu_ph = 770.291 * ((kAe / r) * (kAe / r));
     // maxwell_distribution of the plasma (:353-370): its prefactor
-    const double fpre = P.rho / (kPi * sqrt(kPi) * P.cp * P.cp * P.cp), cp2 = P.cp * P.cp;
+    const double fpre = P.rho / (kPi * sqrt(kPi) * P.cp * P.cp * P.cp), icp2 = 1.0 / (P.cp * P.cp);
 #pragma unroll
     for (int q = 0; q < SPL; q++) {
       if (!mine[q]) continue;
@@ -364,10 +369,10 @@ __global__ __launch_bounds__(256) void k_ena_march(Args a) {
       dv = wy - H4.v;
       const double nu_4 = rate(sqrt(du * du + dv * dv + 0.0 * 0.0), H4);
       const double nu_H = nu_3 + nu_4;
-      S[q] = S[q] * exp(-(nu_ex + nu_ph) * ds / v[q]);
+      S[q] = S[q] * exp(-(nu_ex + nu_ph) * dsv[q]);
       const double mx = wx - P.u, my = wy - P.v, mz = 0.0 - 0.0;
       const double v_sq = mx * mx + my * my + mz * mz;
-      const double f = fpre * exp(-v_sq / cp2);
+      const double f = fpre * exp(-v_sq * icp2);
       j[q] = j[q] + nu_H * f * v[q] * S[q] * ds;
     }
     x0 = x0 + lx * ds;
