@@ -163,9 +163,10 @@ int setmhd_set_dt(setmhd_ctx *ctx, double dt);
 int setmhd_dt_from_state(setmhd_ctx *ctx, double *dt_out);
 
 /* Advances nsteps steps (cuf_kernel.cuf:780-885 without the dumps): dt of
- * step n is the face minimum found in step n-1 (lagged, :789-790).  Runs
- * asynchronously on the context's stream; the device error word is checked
- * when the steps have finished. */
+ * step n is the face minimum found in step n-1 (lagged, :789-790).  All
+ * steps are queued on the context's stream with no host synchronisation
+ * between them (dt never leaves the device); the call returns when they
+ * have finished, with the device error word checked once. */
 int setmhd_step(setmhd_ctx *ctx, int32_t nsteps);
 
 /* As setmhd_step, bracketed by HIP events on the context's stream;
