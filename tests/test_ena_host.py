@@ -99,3 +99,39 @@ def test_read_solution(tmp_path):
     assert t == 1.25
     np.testing.assert_array_equal(cc, fm.cell_center)
     np.testing.assert_array_equal(cp, W2)
+
+
+def test_triangulation_is_delaunay_with_consistent_neighbours():
+    """ena.triangulate stands in for CGAL's Delaunay_triangulation_2
+    (SET_ENA.cpp:242-257), which is unique for points in general position:
+    every triangle counterclockwise, no centre strictly inside any
+    triangle's circumcircle, and nb[t, k] the triangle across the edge
+    opposite vertex k, sharing exactly that edge (-1 only on the hull)."""
+    rng = np.random.default_rng(21)
+    pts = np.c_[rng.uniform(-1, 1, 600), rng.uniform(0, 1, 600), np.zeros(600)]
+    tri, nb = ena.triangulate(pts)
+    p = pts[:, :2]
+    a, b, c = p[tri[:, 0]], p[tri[:, 1]], p[tri[:, 2]]
+    orient = (b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (c[:, 0] - a[:, 0]) * (b[:, 1] - a[:, 1])
+    assert np.all(orient > 0)
+    # in-circle determinant of every point against every triangle (ccw: > 0 means inside)
+    for lo in range(0, len(tri), 200):
+        A, B, C = a[lo:lo + 200, None, :], b[lo:lo + 200, None, :], c[lo:lo + 200, None, :]
+        d = p[None, :, :]
+        m = np.stack([A - d, B - d, C - d], axis=2)  # (T, N, 3, 2)
+        w = (m ** 2).sum(-1)
+        det = (m[..., 0, 0] * (m[..., 1, 1] * w[..., 2] - w[..., 1] * m[..., 2, 1])
+               - m[..., 0, 1] * (m[..., 1, 0] * w[..., 2] - w[..., 1] * m[..., 2, 0])
+               + w[..., 0] * (m[..., 1, 0] * m[..., 2, 1] - m[..., 1, 1] * m[..., 2, 0]))
+        assert det.max() < 1e-12
+    for t in range(len(tri)):
+        for k in range(3):
+            edge = {tri[t, (k + 1) % 3], tri[t, (k + 2) % 3]}
+            o = nb[t, k]
+            if o < 0:
+                continue
+            assert edge <= set(tri[o]) and tri[t, k] not in tri[o]
+            assert t in nb[o]
+    # the hull edges are exactly the edges without a neighbour
+    from scipy.spatial import ConvexHull
+    assert (nb < 0).sum() == len(ConvexHull(p).vertices)
