@@ -163,13 +163,14 @@ def batch_form(precision, steps, seq_steps):
             "sequential_value": cells / seq, "speedup_vs_one_at_a_time": (cells / seq and value / (cells / seq))}
 
 
-def serving_form(n_requests, max_steps, precision=0):
+def serving_form(n_requests, max_steps, precision=0, burst=True):
     """The serving stream through the continuous-batching server and one at a
     time.  work_bound: both modes' cell-update rates over the same work.
     The one-at-a-time rate is over its busy time (the sum of its solves'
     durations), so it is what it reaches however dense the stream; while
     both modes are bound by the work rather than the arrivals, the
-    solves/hour ratio is the ratio of the two rates."""
+    solves/hour ratio is the ratio of the two rates.  burst: the same
+    requests all arriving at once, the server's capacity."""
     from set_amd import serve, workloads
     reqs = workloads.serving_requests(n_requests)
     states = [r.state() for r in reqs]
@@ -180,13 +181,23 @@ def serving_form(n_requests, max_steps, precision=0):
     work_3d = sum(d["cells"] * d["steps"] for d, r in zip(sorted(done_q, key=lambda d: d["rid"]),
                                                             sorted(reqs, key=lambda r: r.rid))
                   if r.grid.nz > 1)
-    return {"requests": n_requests, "max_steps": max_steps, "batched": b, "one_at_a_time": q,
-            "solves_per_hour_ratio": b["solves_per_hour"] / q["solves_per_hour"],
-            "work_bound": {"cell_updates": work, "frac_3d": work_3d / work if work else None,
-                           "last_arrival_s": max(r.arrival for r in reqs),
-                           "batched_rate": work / b["wall_s"],
-                           "one_at_a_time_busy_rate": q["cell_updates"] / busy_q,
-                           "rate_ratio": (work / b["wall_s"]) / (q["cell_updates"] / busy_q)}}
+    out = {"requests": n_requests, "max_steps": max_steps, "batched": b, "one_at_a_time": q,
+           "solves_per_hour_ratio": b["solves_per_hour"] / q["solves_per_hour"],
+           "work_bound": {"cell_updates": work, "frac_3d": work_3d / work if work else None,
+                          "last_arrival_s": max(r.arrival for r in reqs),
+                          "batched_rate": work / b["wall_s"],
+                          "one_at_a_time_busy_rate": q["cell_updates"] / busy_q,
+                          "rate_ratio": (work / b["wall_s"]) / (q["cell_updates"] / busy_q)}}
+    if burst:
+        # the same requests all arriving at once: the server's capacity, which
+        # the stream's arrivals do not bound; one at a time is busy all along
+        # either way, so its busy rate is its capacity
+        import dataclasses
+        now = [dataclasses.replace(r, arrival=0.0) for r in reqs]
+        _, bb = serve.Server(precision=precision, max_steps=max_steps).run(now, states)
+        out["burst"] = {"batched": bb, "batched_rate": bb["cell_updates"] / bb["wall_s"],
+                        "capacity_ratio": (bb["cell_updates"] / bb["wall_s"]) / (q["cell_updates"] / busy_q)}
+    return out
 
 
 def small_form(nx=400, ny=200, steps=500):
@@ -366,7 +377,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--batch-steps", type=int, default=1000)
-    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-max-steps", type=int, default=5000)
     ap.add_argument("--dropin-mesh", default="2000x1000")
     ap.add_argument("--dropin-steps", type=int, default=20)

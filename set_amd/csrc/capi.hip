@@ -83,14 +83,18 @@ struct setmhd_ctx {
   bool mirror_ok = false;
   // overlap of admissions and downloads with a running batch step: a side
   // stream for their copies and conversions, pinned memory for the step's
-  // mirror copy, grow-only device staging (no hipFree while a step runs)
+  // mirror copy, grow-only device staging
   hipStream_t side = nullptr;
   hipEvent_t ev_side = nullptr, ev_step = nullptr;
   bool in_flight = false;
   struct SlotScal* h_mirror = nullptr;  // pinned, slot_cap entries
+  // one staging buffer for admissions' uploads and retirements' downloads,
+  // both on the side stream in its order; sized by the largest grid
+  // admitted, so a download never grows it.  A grown-out buffer is freed
+  // when the GPU is idle (a hipFree would wait for the running step).
   double* up_stage = nullptr;
-  double* dn_stage = nullptr;
-  size_t up_cap = 0, dn_cap = 0;  // doubles
+  size_t up_cap = 0;  // doubles
+  std::vector<double*> stage_old;
   hipGraph_t graph1 = nullptr, graphN = nullptr;
   hipGraphExec_t exec1 = nullptr, execN = nullptr;
   int graph_steps = 8;
@@ -193,10 +197,11 @@ void free_batch(setmhd_ctx* c) {
   c->slot_cap = 0;
   if (c->h_mirror) hipHostFree(c->h_mirror);
   c->h_mirror = nullptr;
-  for (double* p : {c->up_stage, c->dn_stage})
-    if (p) hipFree(p);
-  c->up_stage = c->dn_stage = nullptr;
-  c->up_cap = c->dn_cap = 0;
+  if (c->up_stage) hipFree(c->up_stage);
+  for (double* p : c->stage_old) hipFree(p);
+  c->stage_old.clear();
+  c->up_stage = nullptr;
+  c->up_cap = 0;
   c->in_flight = false;
   c->dirty = true;
   c->tables_dirty = true;
