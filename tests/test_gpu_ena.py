@@ -151,3 +151,24 @@ def test_short_lines_one_segment():
         ref, nref = oracle.ena_los(cc, W, tri, th, 64)
         assert n[a] == nref
         np.testing.assert_allclose(j[a], ref, rtol=0, atol=1e-10 * max(ref.max(), 1e-300))
+
+
+@pytest.mark.parametrize("n_v", [1, 65, 129, 257, 513])
+def test_ragged_speed_counts(n_v):
+    """Speed counts that leave lanes of the last wave empty, at every
+    grouping (1, 2, 4, 8 speeds per lane)."""
+    g = mesh.Grid(30, 15, 1, -0.03, 0.03, 0.0, 0.03)
+    cc = g.cell_centers()
+    W = np.zeros((len(cc), 16))
+    W[:, [0, 1, 2, 4]] = [0.2, -1.0, 0.3, 2.0]
+    W[:, 8:12] = [0.1, -2.0, 0.1, 0.4]
+    W[:, 12:16] = [0.05, -1.4, 0.1, 0.8]
+    tri, nb = ena.triangulate(cc)
+    angles = np.deg2rad([40.0, 90.0])
+    case(f"ENA spectra on a 0.06 x 0.03 mesh, {n_v} speeds")
+    j, n = ena.spectra(cc, W, angles, n_v, tri=(tri, nb))
+    assert j.shape == (2, n_v)
+    for a, th in enumerate(angles):
+        ref, nref = oracle.ena_los(cc, W, tri, th, n_v)
+        assert n[a] == nref > 50
+        np.testing.assert_allclose(j[a], ref, rtol=0, atol=1e-10 * ref.max())
