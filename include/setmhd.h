@@ -304,7 +304,7 @@ int setmhd_probe(setmhd_ctx *ctx, int32_t n, const int32_t *cells, double *out, 
 /* One forward-Euler step of a structured box held in the caller's device
  * memory, allocating nothing: no counterpart in the reference; for callers
  * that cannot allocate device memory (HIP graph capture, tight serving
- * loops).  d_in and d_out (distinct, on the current device) hold the FP64
+ * loops).  d_in and d_out (not overlapping, on the current device) hold the FP64
  * conserved state as the structured path keeps it: 8 fields (rho, rho u,
  * rho v, rho w, E, bx, by, bz with b = B/sqrt(4 pi)) of nx*ny*nz doubles
  * each, cell c = i + nx*(j + ny*k); box and cfg as for

@@ -1206,6 +1206,11 @@ extern "C" int setmhd_step_noalloc(const setmhd_config* cfg, int32_t nx, int32_t
                                    void* stream, double* dt_next, double* residual) {
   if (!cfg || !box || !d_in || !d_out || d_in == d_out) return SETMHD_ERR_ARG;
   if (nx < 1 || ny < 1 || nz < 1 || (long long)nx * ny * nz > (1LL << 31) - 1) return SETMHD_ERR_ARG;
+  {  // the step reads d_in while it writes d_out: the two states must not overlap
+    const uintptr_t n = (uintptr_t)nx * ny * nz * 8 * sizeof(double);
+    const uintptr_t i0 = (uintptr_t)d_in, o0 = (uintptr_t)d_out;
+    if (i0 < o0 + n && o0 < i0 + n) return SETMHD_ERR_ARG;
+  }
   if (!(dt > 0.0) || cfg->precision < 0 || cfg->precision > 1 || cfg->kchunk < 0) return SETMHD_ERR_ARG;
   if (cfg->riemann != SETMHD_RIEMANN_HLLD || cfg->recon != 0 || cfg->time_scheme != SETMHD_TIME_EULER ||
       (cfg->flags & (SETMHD_F_ZERO_BW | SETMHD_F_AXISYM | SETMHD_F_SOURCE_MK | SETMHD_F_HELIO_HACKS)))

@@ -91,3 +91,19 @@ def test_step_noalloc_rejects_oversized_grid():
         assert e.value.status == solver.ERR_ARG
     finally:
         buf.free()
+
+
+def test_step_noalloc_rejects_overlapping_states():
+    """The step reads d_in while it writes d_out: states that share any byte
+    are refused before anything runs."""
+    h = hip()
+    g = mesh.Grid(8, 8, 8)
+    nbytes = g.n_cell * 8 * 8
+    buf = DevBuf(h, 2 * nbytes)
+    try:
+        for shift in (8, nbytes - 8):
+            with pytest.raises(solver.SetMHDError) as e:
+                solver.step_noalloc(solver.cartesian_config(), g, buf.p.value, buf.p.value + shift, 1e-3)
+            assert e.value.status == solver.ERR_ARG
+    finally:
+        buf.free()
