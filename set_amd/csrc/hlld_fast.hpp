@@ -9,7 +9,9 @@
 //   * the face normal is a grid axis, so the tangential basis is the other
 //     two axes instead of the B-aligned basis built at :1235-1273 (the flux is
 //     equivariant under rotations about the normal, so only rounding differs);
-//   * each repeated divisor is inverted once (frcp) and multiplied;
+//   * each repeated divisor is inverted once (frcp) and multiplied, and in
+//     FP64 the divisor pairs (mR - mL, SR - SL) and (SR - SM, SL - SM) share
+//     one reciprocal each (frcp2);
 //   * fast speeds arrive precomputed per cell and per axis (cf_cell below);
 //   * B is carried in Heaviside-Lorentz units b = B/sqrt(4 pi), as chlld uses
 //     internally, so no /sqrt(4 pi) and *sqrt(4 pi) per face.
@@ -80,13 +82,10 @@ struct PickSelect {
 template <class R>
 __device__ __forceinline__ void star_tangential(const Side<R>& X, R S, R m, R su, R SM, R bn, R bn2,
                                                 R& v1, R& v2, R& b1, R& b2) {
-  R tv = R(0), tb = R(0);
   const R tt = m * (S - SM) - bn2;
-  if (fabs(tt) > R(1e-9)) {
-    const R it = frcp(tt);
-    tv = (SM - X.vn) * it;
-    tb = (m * su - bn2) * it;
-  }
+  const R it = fabs(tt) > R(1e-9) ? frcp(tt) : R(0);  // tv = tb = 0 below the switch
+  const R tv = (SM - X.vn) * it;
+  const R tb = (m * su - bn2) * it;
   v1 = X.vt1 - bn * X.bt1 * tv;
   v2 = X.vt2 - bn * X.bt2 * tv;
   b1 = X.bt1 * tb;
@@ -118,14 +117,16 @@ __device__ __forceinline__ bool hlld(const Side<R>& L, const Side<R>& Rr, R F[8]
   }
   const R suR = SR - Rr.vn, suL = SL - L.vn;
   const R mR = suR * Rr.rho, mL = suL * L.rho;
-  const R idn = frcp(mR - mL);
+  R idn, isrl;  // 1/(mR - mL), 1/(SR - SL)
+  frcp2(mR - mL, SR - SL, idn, isrl);
   const R SM = (mR * Rr.vn - Rr.pt + L.pt - mL * L.vn) * idn;            // :1343
   const bool left = SM > R(0);
   const Side<R> Q = pick(left);  // the side whose fan holds the face
   const R ptz = (mR * L.pt - mL * Rr.pt + mR * mL * (Rr.vn - L.vn)) * idn;  // :1600
-  const R bn = (SR * Rr.bn - SL * L.bn) * frcp(SR - SL);                  // UZ(6)
+  const R bn = (SR * Rr.bn - SL * L.bn) * isrl;                          // UZ(6)
   const R bn2 = bn * bn;
-  const R idR = frcp(SR - SM), idL = frcp(SL - SM);
+  R idR, idL;
+  frcp2(SR - SM, SL - SM, idR, idL);
   const R rzR = mR * idR, rzL = mL * idL;  // r*su/(S-SM)
 
   // double-star speeds (:1658-1698)
@@ -135,19 +136,24 @@ __device__ __forceinline__ bool hlld(const Side<R>& L, const Side<R>& Rr, R F[8]
   const R SZL = ibn ? SM - abn * irL : SM;
   const R SZR = ibn ? SM + abn * irR : SM;
 
-  // region, sequential tests of :1722-1814 (later wins)
-  int j = 0;
-  if (SZL >= R(0)) j = 2;                   // SL <= 0 holds here
-  if (ibn && SZL <= R(0) && SM >= R(0)) j = 3;
-  if (ibn && SM <= R(0) && SZR >= R(0)) j = 4;
-  if (SZR <= R(0)) j = 5;                   // SR >= 0 holds here
+  // region, sequential tests of :1722-1814 (later wins): region 2 when
+  // SZL >= 0 (SL <= 0 holds here), 3 and 4 as t3, t4, 5 when SZR <= 0 (SR >= 0
+  // holds here).  Only two facts are used: some region matched (no NaN), and
+  // the winner is 3 or 4.
+  const bool t3 = ibn && SZL <= R(0) && SM >= R(0);
+  const bool t4 = ibn && SM <= R(0) && SZR >= R(0);
+  const bool t5 = SZR <= R(0);
+  const bool found = SZL >= R(0) || t3 || t4 || t5;
+  const bool dstar = (t3 || t4) && !t5;
 
-  // single-star state of the chosen side (:1624-1656) and its HLL jump
+  // single-star state of the chosen side (:1624-1656) and its HLL jump.
+  // su, m and rz repeat suL/suR, mL/mR and rzL/rzR's operations on the
+  // chosen side's operands instead of selecting them.
   const R S = left ? SL : SR;
-  const R su = left ? suL : suR;
-  const R m = left ? mL : mR;
+  const R su = S - Q.vn;
+  const R m = su * Q.rho;
   const R id = left ? idL : idR;
-  const R rz = left ? rzL : rzR;
+  const R rz = m * id;
   R vz1, vz2, bz1, bz2;
   star_tangential(Q, S, m, su, SM, bn, bn2, vz1, vz2, bz1, bz2);
   const R sbvz = bn * SM + bz1 * vz1 + bz2 * vz2;
@@ -163,15 +169,15 @@ __device__ __forceinline__ bool hlld(const Side<R>& L, const Side<R>& Rr, R F[8]
   F[6] += S * (bz1 - Q.bt1);
   F[7] += S * (bz2 - Q.bt2);
 
-  if (j == 3 || j == 4) {  // U** - U*, weighted by SZL / SZR
+  if (dstar) {  // U** - U*, weighted by SZL / SZR
     // kept a branch (an empty volatile asm is not speculated): waves whose
     // lanes all sit outside regions 3 and 4 skip it
     asm volatile("");
     // the other side O and its tangential single-star state
     const Side<R> O = pick(!left);
     const R So = left ? SR : SL;
-    const R suO = left ? suR : suL;
-    const R mO = left ? mR : mL;
+    const R suO = So - O.vn;
+    const R mO = suO * O.rho;
     R vo1, vo2, bo1, bo2;
     star_tangential(O, So, mO, suO, SM, bn, bn2, vo1, vo2, bo1, bo2);
     const R sqL = rzL * irL, sqR = rzR * irR;  // sqrt(rho*)
@@ -195,7 +201,7 @@ __device__ __forceinline__ bool hlld(const Side<R>& L, const Side<R>& Rr, R F[8]
     F[7] += SZ * (c2 - bz2);
   }
   F[5] = fbn;
-  return j != 0;
+  return found;
 }
 
 template <class R>
