@@ -4,10 +4,12 @@
 //
 // Same evaluation order as the Fortran text; the including translation unit
 // turns floating-point contraction off so every product and sum is rounded
-// separately, and division and sqrt are the correctly rounded IEEE forms.
-// Returns 0, or -2 where the reference prints and STOPs (:1816-1825).
+// separately, and division and sqrt are the correctly rounded IEEE forms
+// (quot, sqrt_rn of ieee_dev.hpp), a divisor used more than once being
+// inverted once.  Returns 0, or -2 where the reference prints and STOPs
+// (:1816-1825).
 #pragma once
-#include "common.hpp"
+#include "ieee_dev.hpp"
 
 namespace setmhd {
 
@@ -29,7 +31,15 @@ __device__ __forceinline__ double minmod_ref(double x, double y) {
 // Solvers.f90:140-150
 __device__ __forceinline__ double linear_ref(double x1, double t1, double x2, double t2,
                                              double x3, double t3, double y) {
-  double d = minmod_ref((t1 - t2) / (x1 - x2), (t2 - t3) / (x2 - x3));
+  double d = minmod_ref(quot(t1 - t2, x1 - x2), quot(t2 - t3, x2 - x3));
+  return d * (y - x2) + t2;
+}
+
+// linear_ref with the two divisors inverted beforehand (the TVD calls it for
+// five variables on the same three points): the same bits.
+__device__ __forceinline__ double linear_inv(const Recip& i12, const Recip& i23, double x2, double t1, double t2,
+                                             double t3, double y) {
+  double d = minmod_ref(quot(t1 - t2, i12), quot(t2 - t3, i23));
   return d * (y - x2) + t2;
 }
 
@@ -53,6 +63,7 @@ __device__ int chlld_ref(int n_state, double al, double be, double ge, double w,
   int ik, i;
 
   double spi4 = sqrt(kCpi4); /* :1189-1191 */
+  const Recip i_spi4 = recip_of(spi4);
   double eps = 1.e-12;          /* :1193 */
   double epsb = 1.e-06;         /* :1194 */
   double ga = x5 / x3;          /* :1199 */
@@ -60,9 +71,9 @@ __device__ int chlld_ref(int n_state, double al, double be, double ge, double w,
   double wv = w;
 
   double r1 = qqq1[0], u1 = qqq1[1], v1 = qqq1[2], w1 = qqq1[3], p1 = qqq1[4];
-  double bx1 = qqq1[5] / spi4, by1 = qqq1[6] / spi4, bz1 = qqq1[7] / spi4;
+  double bx1 = quot(qqq1[5], i_spi4), by1 = quot(qqq1[6], i_spi4), bz1 = quot(qqq1[7], i_spi4);
   double r2 = qqq2[0], u2 = qqq2[1], v2 = qqq2[2], w2 = qqq2[3], p2 = qqq2[4];
-  double bx2 = qqq2[5] / spi4, by2 = qqq2[6] / spi4, bz2 = qqq2[7] / spi4;
+  double bx2 = quot(qqq2[5], i_spi4), by2 = quot(qqq2[6], i_spi4), bz2 = quot(qqq2[7], i_spi4);
 
   double ro = (r2 + r1) / x2; /* :1225-1232 */
   double ap = (p2 + p1) / x2;
@@ -77,13 +88,14 @@ __device__ int chlld_ref(int n_state, double al, double be, double ge, double w,
   aco[1][0] = be;
   aco[2][0] = ge;
   if (d > eps) { /* :1243-1251 */
-    d = sqrt(d);
-    aco[0][1] = (abx - bk * al) / d;
-    aco[1][1] = (aby - bk * be) / d;
-    aco[2][1] = (abz - bk * ge) / d;
-    aco[0][2] = (aby * ge - abz * be) / d;
-    aco[1][2] = (abz * al - abx * ge) / d;
-    aco[2][2] = (abx * be - aby * al) / d;
+    d = sqrt_rn(d);
+    const Recip i_d = recip_of(d);
+    aco[0][1] = quot(abx - bk * al, i_d);
+    aco[1][1] = quot(aby - bk * be, i_d);
+    aco[2][1] = quot(abz - bk * ge, i_d);
+    aco[0][2] = quot(aby * ge - abz * be, i_d);
+    aco[1][2] = quot(abz * al - abx * ge, i_d);
+    aco[2][2] = quot(abx * be - aby * al, i_d);
   } else { /* :1252-1273 */
     double aix, aiy, aiz;
     if (fabs(al) < fabs(be) && fabs(al) < fabs(ge)) {
@@ -94,13 +106,14 @@ __device__ int chlld_ref(int n_state, double al, double be, double ge, double w,
       aix = x0; aiy = x0; aiz = x1;
     }
     double aik = aix * al + aiy * be + aiz * ge;
-    d = sqrt(x1 - sq(aik));
-    aco[0][1] = (aix - aik * al) / d;
-    aco[1][1] = (aiy - aik * be) / d;
-    aco[2][1] = (aiz - aik * ge) / d;
-    aco[0][2] = (aiy * ge - aiz * be) / d;
-    aco[1][2] = (aiz * al - aix * ge) / d;
-    aco[2][2] = (aix * be - aiy * al) / d;
+    d = sqrt_rn(x1 - sq(aik));
+    const Recip i_d = recip_of(d);
+    aco[0][1] = quot(aix - aik * al, i_d);
+    aco[1][1] = quot(aiy - aik * be, i_d);
+    aco[2][1] = quot(aiz - aik * ge, i_d);
+    aco[0][2] = quot(aiy * ge - aiz * be, i_d);
+    aco[1][2] = quot(aiz * al - aix * ge, i_d);
+    aco[2][2] = quot(aix * be - aiy * al, i_d);
   }
 
   for (i = 0; i < 3; i++) { /* :1275-1280 */
@@ -110,29 +123,30 @@ __device__ int chlld_ref(int n_state, double al, double be, double ge, double w,
     bR[i] = aco[0][i] * bx2 + aco[1][i] * by2 + aco[2][i] * bz2;
   }
 
-  double aaL = bL[0] / sqrt(r1); /* :1282-1289 */
+  const Recip i_r1 = recip_of(r1), i_r2 = recip_of(r2);
+  double aaL = quot(bL[0], sqrt_rn(r1)); /* :1282-1289 */
   double b2L = sq(bL[0]) + sq(bL[1]) + sq(bL[2]);
-  double b21 = b2L / r1;
-  double cL = sqrt(ga * p1 / r1);
-  double qp = sqrt(b21 + cL * (cL + x2 * aaL));
-  double qm = sqrt(b21 + cL * (cL - x2 * aaL));
+  double b21 = quot(b2L, i_r1);
+  double cL = sqrt_rn(quot(ga * p1, i_r1));
+  double qp = sqrt_rn(b21 + cL * (cL + x2 * aaL));
+  double qm = sqrt_rn(b21 + cL * (cL - x2 * aaL));
   double cfL = (qp + qm) / x2;
   double ptL = p1 + b2L / x2;
 
-  double aaR = bR[0] / sqrt(r2); /* :1291-1298 */
+  double aaR = quot(bR[0], sqrt_rn(r2)); /* :1291-1298 */
   double b2R = sq(bR[0]) + sq(bR[1]) + sq(bR[2]);
-  double b22 = b2R / r2;
-  double cR = sqrt(ga * p2 / r2);
-  qp = sqrt(b22 + cR * (cR + x2 * aaR));
-  qm = sqrt(b22 + cR * (cR - x2 * aaR));
+  double b22 = quot(b2R, i_r2);
+  double cR = sqrt_rn(quot(ga * p2, i_r2));
+  qp = sqrt_rn(b22 + cR * (cR + x2 * aaR));
+  qm = sqrt_rn(b22 + cR * (cR - x2 * aaR));
   double cfR = (qp + qm) / x2;
   double ptR = p2 + b2R / x2;
 
   double aC = (aaL + aaR) / x2; /* :1300-1306 */
   double b2o = (b22 + b21) / x2;
-  double cC = sqrt(ga * ap / ro);
-  qp = sqrt(b2o + cC * (cC + x2 * aC));
-  qm = sqrt(b2o + cC * (cC - x2 * aC));
+  double cC = sqrt_rn(quot(ga * ap, ro));
+  qp = sqrt_rn(b2o + cC * (cC + x2 * aC));
+  qm = sqrt_rn(b2o + cC * (cC - x2 * aC));
   double cfC = (qp + qm) / x2;
   double vC1 = (vL[0] + vR[0]) / x2;
 
@@ -157,8 +171,8 @@ __device__ int chlld_ref(int n_state, double al, double be, double ge, double w,
 
   double suR = SR - vR[0]; /* :1341-1344 */
   double suL = SL - vL[0];
-  double SM = (suR * r2 * vR[0] - ptR + ptL - suL * r1 * vL[0]) /
-              (suR * r2 - suL * r1);
+  const Recip i_dm = recip_of(suR * r2 - suL * r1);  // SM's divisor, and ptz's (HLLD)
+  double SM = quot(suR * r2 * vR[0] - ptR + ptL - suL * r1 * vL[0], i_dm);
 
   *dsl = SL; /* :1346-1348: captured before the LAX override */
   *dsc = SM;
@@ -175,8 +189,9 @@ __device__ int chlld_ref(int n_state, double al, double be, double ge, double w,
   double sbv1 = u1 * bx1 + v1 * by1 + w1 * bz1;
   double upt2 = (sq(u2) + sq(v2) + sq(w2)) / x2;
   double sbv2 = u2 * bx2 + v2 * by2 + w2 * bz2;
-  double e1 = p1 / g1 + r1 * upt1 + b2L / x2;
-  double e2 = p2 / g1 + r2 * upt2 + b2R / x2;
+  const Recip i_g1 = recip_of(g1);
+  double e1 = quot(p1, i_g1) + r1 * upt1 + b2L / x2;
+  double e2 = quot(p2, i_g1) + r2 * upt2 + b2R / x2;
 
   FL[0] = r1 * vL[0]; /* :1377-1393 */
   FL[1] = r1 * vL[0] * vL[0] + ptL - sq(bL[0]);
@@ -207,8 +222,9 @@ __device__ int chlld_ref(int n_state, double al, double be, double ge, double w,
     UR[ik + 5] = bR[ik];
   }
 
+  const Recip i_srl = recip_of(SR - SL);
   for (ik = 0; ik < 8; ik++) /* :1407-1409 */
-    UZ[ik] = (SR * UR[ik] - SL * UL[ik] + FL[ik] - FR[ik]) / (SR - SL);
+    UZ[ik] = quot(SR * UR[ik] - SL * UL[ik] + FL[ik] - FR[ik], i_srl);
 
   if (n_state <= 1) { /* LAX / HLL, :1415-1463 */
     for (ik = 0; ik < 8; ik++) dq[ik] = UR[ik] - UL[ik];
@@ -224,13 +240,13 @@ __device__ int chlld_ref(int n_state, double al, double be, double ge, double w,
       for (ik = 0; ik < 8; ik++) FW[ik] = wv * UR[ik];
     }
     double a = TR * TL;
-    double b = TR - TL;
-    qqq[0] = (TR * FL[0] - TL * FR[0] + a * dq[0]) / b - FW[0];
-    qqq[4] = (TR * FL[4] - TL * FR[4] + a * dq[4]) / b - FW[4];
+    const Recip i_b = recip_of(TR - TL);
+    qqq[0] = quot(TR * FL[0] - TL * FR[0] + a * dq[0], i_b) - FW[0];
+    qqq[4] = quot(TR * FL[4] - TL * FR[4] + a * dq[4], i_b) - FW[4];
     for (ik = 1; ik < 4; ik++)
-      qv[ik - 1] = (TR * FL[ik] - TL * FR[ik] + a * dq[ik]) / b - FW[ik];
+      qv[ik - 1] = quot(TR * FL[ik] - TL * FR[ik] + a * dq[ik], i_b) - FW[ik];
     for (ik = 5; ik < 8; ik++)
-      qb[ik - 5] = (TR * FL[ik] - TL * FR[ik] + a * dq[ik]) / b - FW[ik];
+      qb[ik - 5] = quot(TR * FL[ik] - TL * FR[ik] + a * dq[ik], i_b) - FW[ik];
     for (i = 0; i < 3; i++) {
       qqq[i + 1] = aco[i][0] * qv[0] + aco[i][1] * qv[1] + aco[i][2] * qv[2];
       qqq[i + 5] = aco[i][0] * qb[0] + aco[i][1] * qb[1] + aco[i][2] * qb[2];
@@ -240,8 +256,9 @@ __device__ int chlld_ref(int n_state, double al, double be, double ge, double w,
   }
 
   if (n_state == 2) { /* HLLC, :1465-1596 */
-    double suRm = suR / (SR - SM);
-    double suLm = suL / (SL - SM);
+    const Recip i_srm = recip_of(SR - SM), i_slm = recip_of(SL - SM);
+    double suRm = quot(suR, i_srm);
+    double suLm = quot(suL, i_slm);
     double rzR = r2 * suRm;
     double rzL = r1 * suLm;
     vzR[0] = SM;
@@ -252,25 +269,27 @@ __device__ int chlld_ref(int n_state, double al, double be, double ge, double w,
     bzR[0] = UZ[5];
     bzL[0] = UZ[5];
 
-    vzR[1] = UZ[2] / UZ[0];
-    vzR[2] = UZ[3] / UZ[0];
+    const Recip i_uz0 = recip_of(UZ[0]);
+    vzR[1] = quot(UZ[2], i_uz0);
+    vzR[2] = quot(UZ[3], i_uz0);
     vzL[1] = vzR[1];
     vzL[2] = vzR[2];
 
-    vzR[1] = vR[1] + UZ[5] * (bR[1] - UZ[6]) / suR / r2;
-    vzR[2] = vR[2] + UZ[5] * (bR[2] - UZ[7]) / suR / r2;
-    vzL[1] = vL[1] + UZ[5] * (bL[1] - UZ[6]) / suL / r1;
-    vzL[2] = vL[2] + UZ[5] * (bL[2] - UZ[7]) / suL / r1;
+    const Recip i_sur = recip_of(suR), i_sul = recip_of(suL);
+    vzR[1] = vR[1] + quot(quot(UZ[5] * (bR[1] - UZ[6]), i_sur), i_r2);
+    vzR[2] = vR[2] + quot(quot(UZ[5] * (bR[2] - UZ[7]), i_sur), i_r2);
+    vzL[1] = vL[1] + quot(quot(UZ[5] * (bL[1] - UZ[6]), i_sul), i_r1);
+    vzL[2] = vL[2] + quot(quot(UZ[5] * (bL[2] - UZ[7]), i_sul), i_r1);
 
     bzR[1] = UZ[6];
     bzR[2] = UZ[7];
     bzL[1] = bzR[1];
     bzL[2] = bzR[2];
 
-    double sbvz = (UZ[5] * UZ[1] + UZ[6] * UZ[2] + UZ[7] * UZ[3]) / UZ[0];
+    double sbvz = quot(UZ[5] * UZ[1] + UZ[6] * UZ[2] + UZ[7] * UZ[3], i_uz0);
 
-    double ezR = e2 * suRm + (ptz * SM - ptR * vR[0] + UZ[5] * (sbv2 - sbvz)) / (SR - SM);
-    double ezL = e1 * suLm + (ptz * SM - ptL * vL[0] + UZ[5] * (sbv1 - sbvz)) / (SL - SM);
+    double ezR = e2 * suRm + quot(ptz * SM - ptR * vR[0] + UZ[5] * (sbv2 - sbvz), i_srm);
+    double ezL = e1 * suLm + quot(ptz * SM - ptL * vL[0] + UZ[5] * (sbv1 - sbvz), i_slm);
 
     if (fabs(UZ[5]) < epsb) { /* :1499-1508 */
       vzR[1] = vR[1];
@@ -342,15 +361,15 @@ __device__ int chlld_ref(int n_state, double al, double be, double ge, double w,
   }
 
   if (n_state == 3) { /* HLLD, :1598-1855 */
-    double ptz = (suR * r2 * ptL - suL * r1 * ptR + r1 * r2 * suR * suL * (vR[0] - vL[0])) /
-                 (suR * r2 - suL * r1);
+    double ptz = quot(suR * r2 * ptL - suL * r1 * ptR + r1 * r2 * suR * suL * (vR[0] - vL[0]), i_dm);
     vzL[0] = SM;
     vzR[0] = SM;
     vzzL[0] = SM;
     vzzR[0] = SM;
 
-    double suRm = suR / (SR - SM);
-    double suLm = suL / (SL - SM);
+    const Recip i_srm = recip_of(SR - SM), i_slm = recip_of(SL - SM);
+    double suRm = quot(suR, i_srm);
+    double suLm = quot(suL, i_slm);
     double rzR = r2 * suRm;
     double rzL = r1 * suLm;
 
@@ -366,16 +385,18 @@ __device__ int chlld_ref(int n_state, double al, double be, double ge, double w,
       tvR = x0;
       tbR = x0;
     } else {
-      tvR = (SM - vR[0]) / ttR;
-      tbR = (r2 * suR * suR - bn2) / ttR;
+      const Recip i_tt = recip_of(ttR);
+      tvR = quot(SM - vR[0], i_tt);
+      tbR = quot(r2 * suR * suR - bn2, i_tt);
     }
     double ttL = r1 * suL * (SL - SM) - bn2, tvL, tbL;
     if (fabs(ttL) <= 1.e-09) {
       tvL = x0;
       tbL = x0;
     } else {
-      tvL = (SM - vL[0]) / ttL;
-      tbL = (r1 * suL * suL - bn2) / ttL;
+      const Recip i_tt = recip_of(ttL);
+      tvL = quot(SM - vL[0], i_tt);
+      tbL = quot(r1 * suL * suL - bn2, i_tt);
     }
 
     vzL[1] = vL[1] - bn * bL[1] * tvL; /* :1642-1650 */
@@ -390,23 +411,23 @@ __device__ int chlld_ref(int n_state, double al, double be, double ge, double w,
     double sbvL = bzL[0] * vzL[0] + bzL[1] * vzL[1] + bzL[2] * vzL[2];
     double sbvR = bzR[0] * vzR[0] + bzR[1] * vzR[1] + bzR[2] * vzR[2];
 
-    double ezR = e2 * suRm + (ptz * SM - ptR * vR[0] + bn * (sbv2 - sbvR)) / (SR - SM);
-    double ezL = e1 * suLm + (ptz * SM - ptL * vL[0] + bn * (sbv1 - sbvL)) / (SL - SM);
+    double ezR = e2 * suRm + quot(ptz * SM - ptR * vR[0] + bn * (sbv2 - sbvR), i_srm);
+    double ezL = e1 * suLm + quot(ptz * SM - ptL * vL[0] + bn * (sbv1 - sbvL), i_slm);
 
     double rzzR = rzR; /* :1658-1666 */
     double rzzL = rzL;
-    double rzRs = sqrt(rzR);
-    double rzLs = sqrt(rzL);
+    double rzRs = sqrt_rn(rzR);
+    double rzLs = sqrt_rn(rzL);
     double rzss = rzRs + rzLs;
     double rzps = rzRs * rzLs;
 
-    double SZL = SM - fabs(bn) / rzLs;
-    double SZR = SM + fabs(bn) / rzRs;
+    double SZL = SM - quot(fabs(bn), rzLs);
+    double SZR = SM + quot(fabs(bn), rzRs);
 
     int ibn; /* :1668-1677 */
     double sbn;
     if (fabs(bn) > epsb) {
-      sbn = fabs(bn) / bn;
+      sbn = quot(fabs(bn), bn);
       ibn = 1;
     } else {
       sbn = x0;
@@ -415,13 +436,14 @@ __device__ int chlld_ref(int n_state, double al, double be, double ge, double w,
       SZR = SM;
     }
 
-    vzzL[1] = (rzLs * vzL[1] + rzRs * vzR[1] + sbn * (bzR[1] - bzL[1])) / rzss;
-    vzzL[2] = (rzLs * vzL[2] + rzRs * vzR[2] + sbn * (bzR[2] - bzL[2])) / rzss;
+    const Recip i_rzss = recip_of(rzss);
+    vzzL[1] = quot(rzLs * vzL[1] + rzRs * vzR[1] + sbn * (bzR[1] - bzL[1]), i_rzss);
+    vzzL[2] = quot(rzLs * vzL[2] + rzRs * vzR[2] + sbn * (bzR[2] - bzL[2]), i_rzss);
     vzzR[1] = vzzL[1];
     vzzR[2] = vzzL[2];
 
-    bzzL[1] = (rzLs * bzR[1] + rzRs * bzL[1] + sbn * rzps * (vzR[1] - vzL[1])) / rzss;
-    bzzL[2] = (rzLs * bzR[2] + rzRs * bzL[2] + sbn * rzps * (vzR[2] - vzL[2])) / rzss;
+    bzzL[1] = quot(rzLs * bzR[1] + rzRs * bzL[1] + sbn * rzps * (vzR[1] - vzL[1]), i_rzss);
+    bzzL[2] = quot(rzLs * bzR[2] + rzRs * bzL[2] + sbn * rzps * (vzR[2] - vzL[2]), i_rzss);
     bzzR[1] = bzzL[1];
     bzzR[2] = bzzL[2];
 
