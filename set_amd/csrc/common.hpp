@@ -78,6 +78,24 @@ __device__ __forceinline__ void frcp2(float a, float b, float& ia, float& ib) {
   ib = frcp(b);
 }
 
+// 1/a, 1/b and 1/c from one reciprocal, r = 1/(a b c): 1/a = b c r,
+// 1/b = a c r, 1/c = a b r (within about 3 ulp; a b c must stay in range, as
+// the HLLD's wave-speed differences and switch denominators do).  FP32 keeps
+// three hardware reciprocals.
+__device__ __forceinline__ void frcp3(double a, double b, double c, double& ia, double& ib, double& ic) {
+  const double ab = a * b;
+  const double r = frcp(ab * c);
+  const double rc = r * c;  // 1/(a b)
+  ia = b * rc;
+  ib = a * rc;
+  ic = ab * r;
+}
+__device__ __forceinline__ void frcp3(float a, float b, float c, float& ia, float& ib, float& ic) {
+  ia = frcp(a);
+  ib = frcp(b);
+  ic = frcp(c);
+}
+
 // sqrt of a positive normal number: g = x y ~ sqrt(x) from the estimate y of
 // 1/sqrt(x), e = 1 - g y, sqrt(x) = g (1 + e/2 + 3e^2/8 + ...).
 __device__ __forceinline__ double fsqrt(double x) {

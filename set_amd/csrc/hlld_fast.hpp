@@ -10,8 +10,13 @@
 //     two axes instead of the B-aligned basis built at :1235-1273 (the flux is
 //     equivariant under rotations about the normal, so only rounding differs);
 //   * each repeated divisor is inverted once (frcp) and multiplied, and in
-//     FP64 the divisor pairs (mR - mL, SR - SL) and (SR - SM, SL - SM) share
-//     one reciprocal each (frcp2);
+//     FP64 divisors known at the same point share one reciprocal: the pair
+//     (mR - mL, SR - SL), the triple (S - SM, So - SM, tt) of the side whose
+//     fan holds the face (S), the other side (So) and the single-star switch
+//     denominator, and in the double-star regions the other side's tt with
+//     sqrt(rhoL*) + sqrt(rhoR*) (frcp2, frcp3);
+//   * 1/sqrt(rho*) is formed for the chosen side only, and for the other side
+//     inside the double-star branch, the one place it is used;
 //   * fast speeds arrive precomputed per cell and per axis (cf_cell below);
 //   * B is carried in Heaviside-Lorentz units b = B/sqrt(4 pi), as chlld uses
 //     internally, so no /sqrt(4 pi) and *sqrt(4 pi) per face.
@@ -78,12 +83,11 @@ struct PickSelect {
 
 // Tangential velocity and field of one side's single-star state
 // (:1624-1656): X is the side, m = rho (S - vn), su = S - vn, S its outer
-// wave speed.  The |tt| <= 1e-9 switch leaves the side's tangential state.
+// wave speed, it = 1/tt for tt = m (S - SM) - bn^2, or 0 where
+// |tt| <= 1e-9, which leaves the side's tangential state (tv = tb = 0).
 template <class R>
-__device__ __forceinline__ void star_tangential(const Side<R>& X, R S, R m, R su, R SM, R bn, R bn2,
+__device__ __forceinline__ void star_tangential(const Side<R>& X, R m, R su, R SM, R bn, R bn2, R it,
                                                 R& v1, R& v2, R& b1, R& b2) {
-  const R tt = m * (S - SM) - bn2;
-  const R it = fabs(tt) > R(1e-9) ? frcp(tt) : R(0);  // tv = tb = 0 below the switch
   const R tv = (SM - X.vn) * it;
   const R tb = (m * su - bn2) * it;
   v1 = X.vt1 - bn * X.bt1 * tv;
@@ -125,37 +129,45 @@ __device__ __forceinline__ bool hlld(const Side<R>& L, const Side<R>& Rr, R F[8]
   const R ptz = (mR * L.pt - mL * Rr.pt + mR * mL * (Rr.vn - L.vn)) * idn;  // :1600
   const R bn = (SR * Rr.bn - SL * L.bn) * isrl;                          // UZ(6)
   const R bn2 = bn * bn;
-  R idR, idL;
-  frcp2(SR - SM, SL - SM, idR, idL);
-  const R rzR = mR * idR, rzL = mL * idL;  // r*su/(S-SM)
 
-  // double-star speeds (:1658-1698)
-  const R abn = fabs(bn);
-  const bool ibn = abn > R(1e-6);
-  const R irL = frsq(rzL), irR = frsq(rzR);  // 1/sqrt(rho*)
-  const R SZL = ibn ? SM - abn * irL : SM;
-  const R SZR = ibn ? SM + abn * irR : SM;
-
-  // region, sequential tests of :1722-1814 (later wins): region 2 when
-  // SZL >= 0 (SL <= 0 holds here), 3 and 4 as t3, t4, 5 when SZR <= 0 (SR >= 0
-  // holds here).  Only two facts are used: some region matched (no NaN), and
-  // the winner is 3 or 4.
-  const bool t3 = ibn && SZL <= R(0) && SM >= R(0);
-  const bool t4 = ibn && SM <= R(0) && SZR >= R(0);
-  const bool t5 = SZR <= R(0);
-  const bool found = SZL >= R(0) || t3 || t4 || t5;
-  const bool dstar = (t3 || t4) && !t5;
-
-  // single-star state of the chosen side (:1624-1656) and its HLL jump.
-  // su, m and rz repeat suL/suR, mL/mR and rzL/rzR's operations on the
-  // chosen side's operands instead of selecting them.
-  const R S = left ? SL : SR;
+  // single-star state of the chosen side (:1624-1656).  su and m repeat
+  // suL/suR and mL/mR's operations on the chosen side's operands instead of
+  // selecting them.
+  const R S = left ? SL : SR, So = left ? SR : SL;
   const R su = S - Q.vn;
   const R m = su * Q.rho;
-  const R id = left ? idL : idR;
-  const R rz = m * id;
+  const R tt = m * (S - SM) - bn2;
+  const bool tbig = fabs(tt) > R(1e-9);
+  R id, ido, it;  // 1/(S - SM), 1/(So - SM), 1/tt
+  frcp3(S - SM, So - SM, tbig ? tt : R(1), id, ido, it);
+  it = tbig ? it : R(0);
+  const R rz = m * id;  // rho* = r su/(S - SM)
+
+  // double-star speed of the chosen side (:1658-1698).  The region tests of
+  // :1722-1814 run in order with the later test winning.  Left of the
+  // contact (SM > 0) the right Alfven speed SZR >= SM > 0 rules out regions
+  // 4 and 5, so region 2 (SZL >= 0) or 3 (|bn| > 1e-6, SZL <= 0) wins and
+  // only SZL is needed; otherwise (SM <= 0) SZL <= SM <= 0 makes a match of
+  // regions 2 or 3 imply one of 4 or 5 (SM = 0), which win, and only SZR is
+  // needed.  NaN states match no region.
+  const R abn = fabs(bn);
+  const bool ibn = abn > R(1e-6);
+  const R irz = frsq(rz);  // 1/sqrt(rho*)
+  const R sab = left ? -abn : abn;
+  const R SZ = ibn ? SM + sab * irz : SM;  // SZL or SZR
+  bool found, dstar;
+  if (left) {
+    found = SZ >= R(0) || (ibn && SZ <= R(0));
+    dstar = ibn && SZ <= R(0);
+  } else {
+    const bool t4 = ibn && SM <= R(0) && SZ >= R(0);
+    const bool t5 = SZ <= R(0);
+    found = t4 || t5;
+    dstar = t4 && !t5;
+  }
+
   R vz1, vz2, bz1, bz2;
-  star_tangential(Q, S, m, su, SM, bn, bn2, vz1, vz2, bz1, bz2);
+  star_tangential(Q, m, su, SM, bn, bn2, it, vz1, vz2, bz1, bz2);
   const R sbvz = bn * SM + bz1 * vz1 + bz2 * vz2;
   const R vbQ = Q.vn * Q.bn + Q.vt1 * Q.bt1 + Q.vt2 * Q.bt2;
   const R ez = Q.e * (su * id) + (ptz * SM - Q.pt * Q.vn + bn * (vbQ - sbvz)) * id;
@@ -173,25 +185,27 @@ __device__ __forceinline__ bool hlld(const Side<R>& L, const Side<R>& Rr, R F[8]
     // kept a branch (an empty volatile asm is not speculated): waves whose
     // lanes all sit outside regions 3 and 4 skip it
     asm volatile("");
-    // the other side O and its tangential single-star state
+    // the other side O, its density and tangential single-star state
     const Side<R> O = pick(!left);
-    const R So = left ? SR : SL;
     const R suO = So - O.vn;
     const R mO = suO * O.rho;
+    const R rzO = mO * ido;
+    const R sq = rz * irz, sqo = rzO * frsq(rzO);  // sqrt(rho*)
+    const R ttO = mO * (So - SM) - bn2;
+    const bool tbigO = fabs(ttO) > R(1e-9);
+    R itO, iss;  // 1/ttO, 1/(sqrt(rhoL*) + sqrt(rhoR*))
+    frcp2(tbigO ? ttO : R(1), sq + sqo, itO, iss);
+    itO = tbigO ? itO : R(0);
     R vo1, vo2, bo1, bo2;
-    star_tangential(O, So, mO, suO, SM, bn, bn2, vo1, vo2, bo1, bo2);
-    const R sqL = rzL * irL, sqR = rzR * irR;  // sqrt(rho*)
-    const R sq = left ? sqL : sqR, sqo = left ? sqR : sqL;
+    star_tangential(O, mO, suO, SM, bn, bn2, itO, vo1, vo2, bo1, bo2);
     // sbn (b_R - b_L) = so (b_O - b_Q)
     const R sbn = bn > R(0) ? R(1) : R(-1);
     const R so = left ? sbn : -sbn;
-    const R iss = frcp(sqL + sqR);
     const R w1 = (sq * vz1 + sqo * vo1 + so * (bo1 - bz1)) * iss;
     const R w2 = (sq * vz2 + sqo * vo2 + so * (bo2 - bz2)) * iss;
     const R c1 = (sq * bo1 + sqo * bz1 + so * sq * sqo * (vo1 - vz1)) * iss;
     const R c2 = (sq * bo2 + sqo * bz2 + so * sq * sqo * (vo2 - vz2)) * iss;
     const R sbzz = bn * SM + c1 * w1 + c2 * w2;
-    const R SZ = left ? SZL : SZR;
     // e** = e* -/+ sqrt(rho*) sgn(bn) (b*.v* - b**.v**)
     const R dezz = so * sq * (sbzz - sbvz);
     F[2] += SZ * (rz * (w1 - vz1));

@@ -183,6 +183,10 @@ __device__ __forceinline__ bool held_cell(const SGrid& g, double xc, double yc, 
   return __dsqrt_rn(r2) < g.r_hold;
 }
 
+// The updated state is written with nontemporal stores: it is read again
+// only by the next step, long after it has left L2.
+__device__ __forceinline__ void store_state(double* p, double v) { __builtin_nontemporal_store(v, p); }
+
 // Applies the face sum acc to cell idx, whose state U^n is Uold.  The
 // marches update the row or plane below the current one, whose U^n they keep
 // in registers from the previous iteration.
@@ -194,7 +198,7 @@ __device__ __forceinline__ void finalize_with(const FusedArgs& a, double dt, uns
   if (held) {
     if (MODE != kModeHeun2) {
 #pragma unroll
-      for (int v = 0; v < 8; v++) (a.Uo + v * g.nc)[idx] = Uold[v];
+      for (int v = 0; v < 8; v++) store_state(a.Uo + v * g.nc + idx, Uold[v]);
     }
     return;
   }
@@ -225,7 +229,7 @@ __device__ __forceinline__ void finalize_with(const FusedArgs& a, double dt, uns
   if (!(2.0 * Un[4] * Un[0] - m2 - Un[0] * b2 > 0.0))
     Un[4] = SETMHD_F4(0.0000001) / (kGamma - 1.0) + 0.5 * m2 * frcp(Un[0]) + 0.5 * b2;
 #pragma unroll
-  for (int v = 0; v < 8; v++) (a.Uo + v * g.nc)[idx] = Un[v];
+  for (int v = 0; v < 8; v++) store_state(a.Uo + v * g.nc + idx, Un[v]);
 }
 
 // Raw FP64 state of the own cells of one plane of a tile, [field][row][x],
