@@ -322,6 +322,56 @@ def ena_form(nx, ny, n_v, cpu=True):
     return out
 
 
+def copy_ceiling(nbytes, reps=10):
+    """The device's plain copy rate on this box: a device-to-device
+    hipMemcpy of nbytes (read nbytes + write nbytes, the traffic of one
+    step) through the HIP runtime libsetmhd.so runs on, timed with HIP events
+    after a warm-up.  A practical ceiling beside the nominal HBM peak."""
+    import ctypes
+    from set_amd import solver
+    solver.load()
+    hip = ctypes.CDLL("libamdhip64.so.7")  # the runtime already loaded by libsetmhd.so
+    vp = ctypes.c_void_p
+    hip.hipMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+    hip.hipFree.argtypes = [vp]
+    hip.hipMemcpy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+    hip.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
+    hip.hipEventRecord.argtypes = [vp, vp]
+    hip.hipEventSynchronize.argtypes = [vp]
+    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
+    hip.hipEventDestroy.argtypes = [vp]
+    d2d = 3  # hipMemcpyDeviceToDevice
+
+    def ok(st):
+        if st != 0:
+            raise RuntimeError(f"HIP error {st}")
+    a, b, e0, e1 = vp(), vp(), vp(), vp()
+    ok(hip.hipMalloc(ctypes.byref(a), nbytes))
+    try:
+        ok(hip.hipMalloc(ctypes.byref(b), nbytes))
+        try:
+            ok(hip.hipEventCreate(ctypes.byref(e0)))
+            ok(hip.hipEventCreate(ctypes.byref(e1)))
+            for _ in range(3):
+                ok(hip.hipMemcpy(b, a, nbytes, d2d))
+            ok(hip.hipEventRecord(e0, None))
+            for _ in range(reps):
+                ok(hip.hipMemcpy(b, a, nbytes, d2d))
+            ok(hip.hipEventRecord(e1, None))
+            ok(hip.hipEventSynchronize(e1))
+            ms = ctypes.c_float()
+            ok(hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1))
+            hip.hipEventDestroy(e0)
+            hip.hipEventDestroy(e1)
+        finally:
+            hip.hipFree(b)
+    finally:
+        hip.hipFree(a)
+    t = ms.value / reps
+    return {"GBs": 2 * nbytes / (t * 1e-3) / 1e9, "ms": t, "bytes_each_way": nbytes,
+            "how": "hipMemcpy device to device of one step's state, average of 10 after 3 warm-up"}
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -383,6 +433,8 @@ def main():
     ap.add_argument("--dropin-steps", type=int, default=20)
     ap.add_argument("--ena-mesh", default="400x200")
     ap.add_argument("--ena-speeds", type=int, default=1000)
+    ap.add_argument("--no-copy-ceiling", action="store_true",
+                    help="skip the device-copy ceiling measurement")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -459,6 +511,14 @@ def main():
                 "frac_wall": fp["achieved_GBs_wall"] / PEAK_HBM_GBS,
                 "kernel": "k_fused<double,16,16,3,Euler> (one launch per step)",
                 "traffic_source": f"profiles/fused_fp64_euler_{n}cube.txt" if traffic else None}
+    # this box's plain copy rate over the same bytes: the practical ceiling
+    if not args.no_copy_ceiling:
+        try:
+            cc = copy_ceiling(g.n_cell * 64)
+            roofline["copy_ceiling"] = {**cc, "frac_of_peak": cc["GBs"] / PEAK_HBM_GBS,
+                                        "kernel_frac_of_copy": achieved / cc["GBs"]}
+        except Exception as e:  # the headline line must not depend on it
+            roofline["copy_ceiling"] = {"error": repr(e)}
     # FP64 flops of the same kernel (VALU counters of the committed profile)
     fl = from_profiles(f"fused_fp64_euler_{n}cube", "fp64_flops_per_cell_update")
     if fl:
