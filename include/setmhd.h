@@ -333,7 +333,8 @@ int setmhd_step_noalloc(const setmhd_config *cfg, int32_t nx, int32_t ny, int32_
  * clockwise) a triangulation of the cell centres -- the reference builds a
  * CGAL Delaunay triangulation (:242-257) -- and tri_nb[3 t + k] the triangle
  * across the edge opposite vertex k (-1 on the hull).  Synchronous on
- * `device`; allocates its own device memory.  SETMHD_ERR_ARG also when a
+ * `device` (the calling thread's current device is left as it was);
+ * allocates its own device memory.  SETMHD_ERR_ARG also when a
  * walk between triangles does not end (inconsistent neighbours). */
 int setmhd_ena_spectra(int32_t device, int32_t n_cell, const double *cell_center, const double *cell_par,
                        int32_t n_tri, const int32_t *tri, const int32_t *tri_nb, int32_t n_angle,

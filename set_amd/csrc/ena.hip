@@ -418,7 +418,13 @@ extern "C" int setmhd_ena_spectra(int32_t device, int32_t n_cell, const double* 
     return SETMHD_ERR_ARG;
   for (long long k = 0; k < 3LL * n_tri; k++)
     if (tri[k] < 0 || tri[k] >= n_cell || tri_nb[k] < -1 || tri_nb[k] >= n_tri) return SETMHD_ERR_ARG;
-  if (hipSetDevice(device) != hipSuccess) return SETMHD_ERR_HIP;
+  int prev_device = 0;
+  if (hipGetDevice(&prev_device) != hipSuccess || hipSetDevice(device) != hipSuccess) return SETMHD_ERR_HIP;
+  // the caller's current device is restored on every return below
+  struct Restore {
+    int d;
+    ~Restore() { (void)hipSetDevice(d); }
+  } restore{prev_device};
   const size_t b_cc = 3 * (size_t)n_cell * 8, b_par = 16 * (size_t)n_cell * 8, b_tri = 3 * (size_t)n_tri * 4;
   const size_t b_ang = (size_t)n_angle * 8, b_out = (size_t)n_angle * n_v * 8, b_smp = (size_t)n_angle * 4;
   ena::Args a{};
