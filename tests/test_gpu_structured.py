@@ -180,6 +180,26 @@ def test_heun_parity():
     assert np.all(e <= 1e-10), e
 
 
+def test_heun_parity_3d():
+    """Heun RK2 (two launches per step, stage 2 averaging in place) in 3-D
+    against the oracle: ragged 16x16 tiles, 60 steps; rho, m, E to 1e-10,
+    the field components to 10x their one-ulp sensitivity."""
+    case("structured 3-D 50x42x36 Heun RK2 FP64 steps=60")
+    g = mesh.grid_3d(50, 42, 36)
+    W = mesh.heliosphere_state(g, wind=mesh.Wind(r_in=0.3))
+    s, _ = oracle_run(g, W, 60, time_scheme=1)
+    Wp = W.copy()
+    Wp[:, :8] = np.nextafter(Wp[:, :8], np.inf)
+    sp, _ = oracle_run(g, Wp, 60, time_scheme=1)
+    sens = rel_err(cons(sp.W), cons(s.W), "l1")
+    cp, t, _ = gpu_run(g, W, 60, time_scheme=1)
+    e = rel_err(cons(cp), cons(s.W), "l1")
+    print("rel L1 per var", e, "1-ulp sensitivity", sens)
+    assert t == pytest.approx(s.t, rel=1e-11)
+    assert np.all(e[:5] <= 1e-10), e
+    assert np.all(e[5:] <= np.maximum(10.0 * sens[5:], 1e-10)), (e, sens)
+
+
 def test_mixed_precision_error():
     case("structured 2-D 400x200 mixed vs FP64 steps=500")
     g = mesh.grid_2d(400, 200)
