@@ -145,7 +145,7 @@ struct PickLds {
 // face (the face on the low side of cell (ry, rx)).
 template <int D, class R, int TX, int TY>
 __device__ __forceinline__ void lds_face(Smem<R, TX, TY>& sm, int ly, int lx, int ry, int rx,
-                                         R& smax, bool& ok) {
+                                         R& smax, bool& ok, R out[8]) {
   constexpr int D1 = D + 1, D2 = D == 0 ? 2 : 0;
   const int offL = ly * (TX + 2) + lx, offR = ry * (TX + 2) + rx;
   const Side<R> L = get_side<D>(sm, offL);
@@ -157,15 +157,17 @@ __device__ __forceinline__ void lds_face(Smem<R, TX, TY>& sm, int ly, int lx, in
     if constexpr (D == 0) return sm.fx;
     else return sm.fy;
   }();
+  out[0] = F[0];
+  out[1 + D] = F[1];
+  out[1 + D1] = F[2];
+  out[1 + D2] = F[3];
+  out[4] = F[4];
+  out[5 + D] = F[5];
+  out[5 + D1] = F[6];
+  out[5 + D2] = F[7];
   const int fy = ry - 1, fx = rx - 1;
-  dst[0][fy][fx] = F[0];
-  dst[1 + D][fy][fx] = F[1];
-  dst[1 + D1][fy][fx] = F[2];
-  dst[1 + D2][fy][fx] = F[3];
-  dst[4][fy][fx] = F[4];
-  dst[5 + D][fy][fx] = F[5];
-  dst[5 + D1][fy][fx] = F[6];
-  dst[5 + D2][fy][fx] = F[7];
+#pragma unroll
+  for (int v = 0; v < 8; v++) dst[v][fy][fx] = out[v];
 }
 
 // The 8 fields of cell idx: uniform (SGPR) field base + 32-bit offset.
@@ -404,6 +406,8 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
     derive<R, true>(Uc, qc);
     const Side<R> zc = side_of(qc, 2);
     if (!below) {
+      // Stage the plane (the previous plane's faces were all read before
+      // its flux barrier).
       if (!top) {
         put_q(sm, ty + 1, tx + 1, qc);
         if (has_h) {
@@ -411,38 +415,18 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
           derive<R, false>(Uh, qh);
           put_q(sm, hly, hlx, qh);
         }
-        __syncthreads();
-        // every wave has read the staged plane: stage plane k+1 (the closing
-        // plane when k+1 == k1)
-        if constexpr (DMA) dma_plane<TX, TY>(*st, a, min(k + 1, g.nz - 1), wid, dma_off);
-
-        // x- and y-faces of the plane.  The TY faces on the tile's far x edge
-        // go to lanes 0..TY-1 of the role NW-1 wave, the TX faces on the far y
-        // edge to lanes 0..TX-1 of the role NW-2 wave, each as a second
-        // iteration of its loop.
-        {
-          const int nx_it = (role == NW - 1 && lane < TY) ? 2 : 1;
-          for (int r = 0; r < nx_it; r++) {
-            const int row = r == 0 ? ty + 1 : lane + 1;
-            const int col = r == 0 ? tx : TX;
-            lds_face<0>(sm, row, col, row, col + 1, smax[0], ok);
-          }
-          const int ny_it = (role == NW - 2 && lane < TX) ? 2 : 1;
-          for (int r = 0; r < ny_it; r++) {
-            const int row = r == 0 ? ty : TY;
-            const int col = r == 0 ? tx + 1 : lane + 1;
-            lds_face<1>(sm, row, col, row + 1, col, smax[1], ok);
-          }
-        }
       }
+      // The z-face below this cell and the update of the cell below, whose
+      // faces are then all known, need registers only: they run before the
+      // staging barrier.
       R Fz[8];
-      {  // z-face below this cell
+      {
         R s;
         ok &= hlld(zp, zc, Fz, s);
         smax[2] = fmax(smax[2], s);
       }
       // z-face: rotate frame (z, x, y) back to x, y, z
-      double fz[8];
+      R fz[8];
       fz[0] = Fz[0];
       fz[3] = Fz[1];
       fz[1] = Fz[2];
@@ -451,27 +435,43 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
       fz[7] = Fz[5];
       fz[5] = Fz[6];
       fz[6] = Fz[7];
-      // DMA: this wave's share of plane k+1 has landed.  Otherwise the
-      // prefetch, waited for here, before the finalize stores: a wait after
-      // them (or at the loop's back edge) would drain those stores as well.
-      wait_vm();
-      if (k > k0 && own) {  // update the cell below: its faces are all known
+      if (k > k0 && own) {
 #pragma unroll
-        for (int v = 0; v < 8; v++) accp[v] = accp[v] + fz[v] * g.az;
+        for (int v = 0; v < 8; v++) accp[v] = accp[v] + (double)fz[v] * g.az;
         const double zcen = __dadd_rn(g.z0, __dmul_rn((double)(k - 1) + 0.5, g.dz));
         finalize_with<MODE>(a, dt, col0 + plane * (unsigned)(k - 1), held_cell(g, xc, yc, zcen), accp, Uprev,
                             rs_d, rs_r);
       }
       if (!top) {
         __syncthreads();
-        // signed face sum in the face-slot order (x-, x+, y-, y+, z-, z+)
+        // every wave has read the staged plane: stage plane k+1 (the closing
+        // plane when k+1 == k1)
+        if constexpr (DMA) dma_plane<TX, TY>(*st, a, min(k + 1, g.nz - 1), wid, dma_off);
+
+        // x- and y-faces of the plane.  Each thread forms the faces on the
+        // low side of its cell and keeps their fluxes (fxo, fyo); the TY faces
+        // on the tile's far x edge go to lanes 0..TY-1 of the role NW-1 wave,
+        // the TX faces on the far y edge to lanes 0..TX-1 of the role NW-2
+        // wave, each as a second iteration of its loop.
+        R fxo[8], fyo[8], fe[8];
+        {
+          lds_face<0>(sm, ty + 1, tx, ty + 1, tx + 1, smax[0], ok, fxo);
+          if (role == NW - 1 && lane < TY) lds_face<0>(sm, lane + 1, TX, lane + 1, TX + 1, smax[0], ok, fe);
+          lds_face<1>(sm, ty, tx + 1, ty + 1, tx + 1, smax[1], ok, fyo);
+          if (role == NW - 2 && lane < TX) lds_face<1>(sm, TY, lane + 1, TY + 1, lane + 1, smax[1], ok, fe);
+        }
+        // DMA: this wave's share of plane k+1 has landed before the barrier
+        if constexpr (DMA) wait_vm();
+        __syncthreads();
+        // signed face sum in the face-slot order (x-, x+, y-, y+, z-, z+);
+        // the x+ and y+ fluxes are the neighbours'
 #pragma unroll
         for (int v = 0; v < 8; v++) {
-          double s = -((double)sm.fx[v][ty][tx] * g.ax);
+          double s = -((double)fxo[v] * g.ax);
           s = s + (double)sm.fx[v][ty][tx + 1] * g.ax;
-          s = s - (double)sm.fy[v][ty][tx] * g.ay;
+          s = s - (double)fyo[v] * g.ay;
           s = s + (double)sm.fy[v][ty + 1][tx] * g.ay;
-          accp[v] = s - fz[v] * g.az;
+          accp[v] = s - (double)fz[v] * g.az;
         }
       }
     }
