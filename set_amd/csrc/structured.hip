@@ -145,11 +145,12 @@ struct PickLds {
 // face (the face on the low side of cell (ry, rx)).
 template <int D, class R, int TX, int TY>
 __device__ __forceinline__ void lds_face(Smem<R, TX, TY>& sm, int ly, int lx, int ry, int rx,
-                                         R& smax, bool& ok, R out[8]) {
+                                         R& smax, bool& ok, R out[8], const CellQ<R>* qr = nullptr) {
   constexpr int D1 = D + 1, D2 = D == 0 ? 2 : 0;
   const int offL = ly * (TX + 2) + lx, offR = ry * (TX + 2) + rx;
   const Side<R> L = get_side<D>(sm, offL);
-  const Side<R> Rr = get_side<D>(sm, offR);
+  // the right cell from registers when the caller holds it (its own cell)
+  const Side<R> Rr = qr ? side_of(*qr, D) : get_side<D>(sm, offR);
   R F[8], s;
   ok &= hlld(L, Rr, F, s, PickLds<D, R, TX, TY>{sm, offL, offR});
   smax = fmax(smax, s);
@@ -455,9 +456,9 @@ __device__ __forceinline__ TileRed fused_tile(const FusedArgs& a, int b, bool bo
         // wave, each as a second iteration of its loop.
         R fxo[8], fyo[8], fe[8];
         {
-          lds_face<0>(sm, ty + 1, tx, ty + 1, tx + 1, smax[0], ok, fxo);
+          lds_face<0>(sm, ty + 1, tx, ty + 1, tx + 1, smax[0], ok, fxo, &qc);
           if (role == NW - 1 && lane < TY) lds_face<0>(sm, lane + 1, TX, lane + 1, TX + 1, smax[0], ok, fe);
-          lds_face<1>(sm, ty, tx + 1, ty + 1, tx + 1, smax[1], ok, fyo);
+          lds_face<1>(sm, ty, tx + 1, ty + 1, tx + 1, smax[1], ok, fyo, &qc);
           if (role == NW - 2 && lane < TX) lds_face<1>(sm, TY, lane + 1, TY + 1, lane + 1, smax[1], ok, fe);
         }
         // DMA: this wave's share of plane k+1 has landed before the barrier
