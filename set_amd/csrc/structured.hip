@@ -152,7 +152,10 @@ __device__ __forceinline__ void lds_face(Smem<R, TX, TY>& sm, int ly, int lx, in
   // the right cell from registers when the caller holds it (its own cell)
   const Side<R> Rr = qr ? side_of(*qr, D) : get_side<D>(sm, offR);
   R F[8], s;
-  ok &= hlld(L, Rr, F, s, PickLds<D, R, TX, TY>{sm, offL, offR});
+  // FP64 re-reads the chosen side from LDS (a select costs two instructions
+  // per value and keeps both sides live); FP32 selects it in registers
+  if constexpr (sizeof(R) == 8) ok &= hlld(L, Rr, F, s, PickLds<D, R, TX, TY>{sm, offL, offR});
+  else ok &= hlld(L, Rr, F, s);
   smax = fmax(smax, s);
   auto& dst = [&]() -> auto& {
     if constexpr (D == 0) return sm.fx;
