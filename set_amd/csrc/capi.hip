@@ -807,8 +807,16 @@ int setmhd_upload_mesh(setmhd_ctx* c, int32_t n_cell, int32_t n_gran, const doub
     for (int k = 0; k < 6; k++) listed |= g[k] == (int32_t)f + 1;
     c->per_cell_ok = listed;
   }
+  // the cell pass's gather table: each Cell_gran slot signed by the cell's
+  // side of the face
+  std::vector<int32_t> gsig(nc * 6);
+  for (size_t i = 0; i < nc; i++)
+    for (int k = 0; k < 6; k++) {
+      const int32_t j = cell_gran[6 * i + k];
+      gsig[6 * i + k] = j == 0 ? 0 : gran_neighbour[2 * (size_t)(j - 1)] == (int32_t)i + 1 ? j : -j;
+    }
   // one block for the geometry (and the cell permutation)
-  size_t bytes = nc * (3 + 1) * 8 + nf * (3 + 1 + 3) * 8 + nc * 6 * 4 + nf * (2 + 2 + 1) * 4 + 64 +
+  size_t bytes = nc * (3 + 1) * 8 + nf * (3 + 1 + 3) * 8 + 2 * nc * 6 * 4 + nf * (2 + 2 + 1) * 4 + 72 +
                  (reorder ? nc * 4 + 8 : 0);
   char* base = nullptr;
   if (hipMalloc((void**)&base, bytes) != hipSuccess) return fail(c, SETMHD_ERR_NOMEM, "hipMalloc mesh");
@@ -825,6 +833,7 @@ int setmhd_upload_mesh(setmhd_ctx* c, int32_t n_cell, int32_t n_gran, const doub
   double* gs = (double*)take(nf * 8);
   double* gc = (double*)take(nf * 3 * 8);
   int* cg = (int*)take(nc * 6 * 4);
+  int* cgs = (int*)take(nc * 6 * 4);
   int* gnb = (int*)take(nf * 2 * 4);
   int* gtv = (int*)take(nf * 2 * 4);
   int* gty = (int*)take(nf * 4);
@@ -835,12 +844,13 @@ int setmhd_upload_mesh(setmhd_ctx* c, int32_t n_cell, int32_t n_gran, const doub
   HIPCHK(c, hipMemcpyAsync(gs, gran_square, nf * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(gc, gran_center, nf * 3 * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(cg, cell_gran, nc * 6 * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(cgs, gsig.data(), nc * 6 * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(gnb, gran_neighbour, nf * 2 * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(gtv, gran_neighbour_tvd, nf * 2 * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(gty, gran_type, nf * 4, hipMemcpyHostToDevice, c->stream));
   if (reorder) HIPCHK(c, hipMemcpyAsync(cpm, cperm.data(), nc * 4, hipMemcpyHostToDevice, c->stream));
   c->d_cperm = cpm;
-  c->um = UMesh{n_cell, n_gran, cc, cv, cg, gn, gs, gc, gnb, gtv, gty};
+  c->um = UMesh{n_cell, n_gran, cc, cv, cg, gn, gs, gc, gnb, gtv, gty, cgs};
   int st;
   if ((st = dalloc(c, &c->d_W, nc * 8)) || (st = dalloc(c, &c->d_W2, nc * 8)) ||
       (st = dalloc(c, &c->d_H, nc * 8)) || (st = dalloc(c, &c->d_potok, nf * 8)))

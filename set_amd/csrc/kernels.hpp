@@ -17,6 +17,10 @@ struct UMesh {
   const int* gran_nb;         // [2*Nf]
   const int* gran_tvd;        // [2*Nf]
   const int* gran_type;       // [Nf]
+  // [6*Nc]: Cell_gran with the cell's side as the sign, +face where the cell
+  // is the face's first neighbour, -face otherwise, 0 for an empty slot (the
+  // cell pass's gather table: no gather of Gran_neighbour per slot)
+  const int* cell_gs;
 };
 
 struct UCfg {
