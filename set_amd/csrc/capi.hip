@@ -158,6 +158,18 @@ int dalloc(setmhd_ctx* c, T** p, size_t n) {
   return SETMHD_OK;
 }
 
+// A temporary device buffer freed on every return path (hipFree waits for
+// the work queued on it).
+struct StageBuf {
+  double* p = nullptr;
+  StageBuf() = default;
+  StageBuf(const StageBuf&) = delete;
+  StageBuf& operator=(const StageBuf&) = delete;
+  ~StageBuf() {
+    if (p) hipFree(p);
+  }
+};
+
 void free_graphs(setmhd_ctx* c) {
   if (c->exec1) hipGraphExecDestroy(c->exec1);
   if (c->execN) hipGraphExecDestroy(c->execN);
@@ -855,10 +867,10 @@ int setmhd_upload_mesh(setmhd_ctx* c, int32_t n_cell, int32_t n_gran, const doub
   if ((st = dalloc(c, &c->d_W, nc * 8)) || (st = dalloc(c, &c->d_W2, nc * 8)) ||
       (st = dalloc(c, &c->d_H, nc * 8)) || (st = dalloc(c, &c->d_potok, nf * 8)))
     return st;
-  double* stage = nullptr;
-  if ((st = dalloc(c, &stage, nc * 16))) return st;
-  HIPCHK(c, hipMemcpyAsync(stage, cell_par, nc * 16 * 8, hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_split16, dim3(nblocks(nc)), dim3(256), 0, c->stream, stage, (long long)nc,
+  StageBuf stage;
+  if ((st = dalloc(c, &stage.p, nc * 16))) return st;
+  HIPCHK(c, hipMemcpyAsync(stage.p, cell_par, nc * 16 * 8, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_split16, dim3(nblocks(nc)), dim3(256), 0, c->stream, stage.p, (long long)nc,
                      c->d_cperm, c->d_W, c->d_H);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(c->d_W2, c->d_W, nc * 8 * 8, hipMemcpyDeviceToDevice, c->stream));  // :361
@@ -874,7 +886,6 @@ int setmhd_upload_mesh(setmhd_ctx* c, int32_t n_cell, int32_t n_gran, const doub
     HIPCHK(c, hipMemsetAsync(c->d_potok, 0, nf * 8 * 8, c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  hipFree(stage);
   c->n_cell = n_cell;
   c->n_gran = n_gran;
   c->cperm = std::move(cperm);
@@ -904,16 +915,15 @@ int setmhd_upload_structured(setmhd_ctx* c, int32_t nx, int32_t ny, int32_t nz, 
     return st;
   // Cell_par (16 doubles per cell) is staged in a temporary device buffer
   // and converted there (conserved state, neutral moments).
-  double* stage = nullptr;
-  if ((st = dalloc(c, &stage, (size_t)nc * 16))) return st;
-  HIPCHK(c, hipMemcpyAsync(stage, cell_par, (size_t)nc * 16 * 8, hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_prim_to_cons, dim3(nblocks(nc)), dim3(256), 0, c->stream, stage, nc, c->d_U);
+  StageBuf stage;
+  if ((st = dalloc(c, &stage.p, (size_t)nc * 16))) return st;
+  HIPCHK(c, hipMemcpyAsync(stage.p, cell_par, (size_t)nc * 16 * 8, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_prim_to_cons, dim3(nblocks(nc)), dim3(256), 0, c->stream, stage.p, nc, c->d_U);
   HIPCHK(c, hipGetLastError());
-  hipLaunchKernelGGL(k_take_h, dim3(nblocks(nc)), dim3(256), 0, c->stream, stage, nc, c->d_H16);
+  hipLaunchKernelGGL(k_take_h, dim3(nblocks(nc)), dim3(256), 0, c->stream, stage.p, nc, c->d_H16);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(c->d_U2, c->d_U, (size_t)nc * 8 * 8, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  hipFree(stage);
   c->sg = make_sgrid(c->cfg, nx, ny, nz, box);
   c->kchunk = grid_kchunk(c, nx, ny, nz, false);
   c->cur_buf = 0;
