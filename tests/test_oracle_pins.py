@@ -570,3 +570,112 @@ def test_source_mk_thermal_closed_form():
         S = oracle.source_mk(two_alike(rho, 0.3, -0.4, p, rh, 0.3, -0.4, th)) / 2
         assert np.abs(S[:2]).max() == 0.0
         np.testing.assert_allclose(S[2], kk * rho * rh * 0.75 * (th - p / rho), rtol=1e-12)
+
+
+# Probes of the heliosphere switches' regions, with the region each lies in
+# written out by hand from the reference text (cuf_solvers.cuf:296-298 for the
+# LAX wedge, :783-785 for the v clamp).  The thresholds are REAL*4 literals
+# compared in REAL*8: 0.22 -> 0.2199999988, 0.1 -> 0.1000000015,
+# 0.28 -> 0.2800000012, 0.07 -> 0.0700000003; the comparisons are strict.
+# Probes between a literal's single and double value fail a double literal;
+# probes on either side of each threshold fail a reversed or non-strict
+# comparison, a swapped axis or the wrong normal component.
+LAX_PROBES = [  # (face centre x, face centre y, normal x, LAX?)
+    (0.5, 0.05, 0.0, True),
+    (0.2200001, 0.05, 0.0, True),
+    (0.22, 0.05, 0.0, True),             # above 0.2199999988
+    (0.2199999995, 0.05, 0.0, True),     # between the single and the double 0.22
+    (0.219999998, 0.05, 0.0, False),
+    (-0.5, 0.05, 0.0, False),
+    (0.5, 0.1000000010, 0.0, True),      # between the double and the single 0.1
+    (0.5, 0.1000000020, 0.0, False),
+    (0.5, -0.3, 0.0, True),              # no absolute value on y
+    (0.05, 0.5, 0.0, False),             # x and y swapped
+    (0.5, 0.05, 0.0999999, True),
+    (0.5, 0.05, 0.1000000010, True),     # between the double and the single 0.1
+    (0.5, 0.05, -0.1000000010, True),    # |n_x|
+    (0.5, 0.05, 0.1000000020, False),
+    (0.5, 0.05, -0.3, False),
+    (float(np.float32(0.22)), 0.05, 0.0, False),   # on each threshold: strict comparisons
+    (0.5, float(np.float32(0.1)), 0.0, False),
+    (0.5, 0.05, float(np.float32(0.1)), False),
+]
+
+CLAMP_PROBES = [  # (cell centre x, cell centre y, v clamped to 0.01?)
+    (0.5, 0.03, True),
+    (0.2800001, 0.03, True),
+    (0.2800000005, 0.03, False),         # above the double 0.28, below the single
+    (0.28, 0.03, False),
+    (0.5, 0.0700000001, True),           # below the single 0.07, above the double
+    (0.5, 0.0700000005, False),
+    (0.5, -0.2, True),                   # no absolute value on y
+    (0.03, 0.5, False),                  # x and y swapped
+    (float(np.float32(0.28)), 0.03, False),  # on each threshold: strict comparisons
+    (0.5, float(np.float32(0.07)), False),
+]
+
+
+def helio_cfg(riemann):
+    cfg = oracle.cartesian_config(riemann=riemann, recon=0)
+    cfg.flags = oracle.F_HELIO_HACKS
+    cfg.r_skip_face = -1.0
+    cfg.r_hold_cell = -1.0
+    return cfg
+
+
+@pytest.mark.parametrize("riemann", [2, 3])
+def test_lax_wedge_region(riemann):
+    """With the heliosphere switches a face takes the LAX flux (n_state 0)
+    exactly where the reference's wedge test holds, and the configured
+    solver elsewhere; each probe face's flux is compared bit for bit with
+    chlld of its two cells through its (possibly tilted) normal."""
+    rng = np.random.default_rng(61)
+    fm = tensor_mesh(np.linspace(0.0, 1.0, 7), np.linspace(0.0, 1.0, 7))
+    inner = np.nonzero(fm.gran_neighbour[:, 1] > 0)[0]
+    assert len(inner) >= len(LAX_PROBES)
+    fm.gran_type[:] = 2
+    fm.gran_type[fm.gran_neighbour[:, 1] < 0] = 1
+    probes = inner[rng.permutation(len(inner))[:len(LAX_PROBES)]]
+    for f, (x, y, nx, _) in zip(probes, LAX_PROBES):
+        fm.gran_center[f] = (x, y, 0.0)
+        fm.gran_normal[f] = (nx, np.sqrt(1.0 - nx * nx), 0.0)
+    W = random_state(rng, fm.n_cell, b=0.5)
+    pot = face_flux(helio_cfg(riemann), fm, W)
+    lax_seen = 0
+    for f, (_, _, _, lax) in zip(probes, LAX_PROBES):
+        q1 = W[fm.gran_neighbour[f, 0] - 1, :8]
+        q2 = W[fm.gran_neighbour[f, 1] - 1, :8]
+        want = oracle.chlld(0 if lax else riemann, fm.gran_normal[f], q1, q2)[0] * fm.gran_square[f]
+        other = oracle.chlld(riemann if lax else 0, fm.gran_normal[f], q1, q2)[0] * fm.gran_square[f]
+        assert not np.array_equal(want, other)  # the two solvers differ on this face
+        np.testing.assert_array_equal(pot[f, :8], want)
+        lax_seen += lax
+    assert 0 < lax_seen < len(LAX_PROBES)
+
+
+def test_v_clamp_region():
+    """With the heliosphere switches the updated v of a cell is set to the
+    REAL*4 0.01 exactly where the reference's tail test holds and v < 0;
+    a uniform state with v < 0 is otherwise preserved (every boundary a copy
+    ghost)."""
+    fm = tensor_mesh(np.linspace(0.0, 1.0, 7), np.linspace(0.0, 1.0, 7))
+    fm.gran_type[:] = 2
+    fm.gran_type[fm.gran_neighbour[:, 1] < 0] = 1
+    cells = np.arange(len(CLAMP_PROBES)) * 2
+    for c, (x, y, _) in zip(cells, CLAMP_PROBES):
+        fm.cell_center[c] = (x, y, 0.0)
+    W = np.zeros((fm.n_cell, 16))
+    W[:, 0], W[:, 1], W[:, 2], W[:, 4] = 1.0, -2.0, -0.5, 1.0
+    # one cell outside the probes carries v > 0 in the tail region: untouched
+    fm.cell_center[1] = (0.5, 0.03, 0.0)
+    W[1, 2] = 0.25
+    s = oracle.Solver(fm, W, helio_cfg(2))
+    st, dt = s.face_pass(0)
+    assert st == 0
+    assert s.cell_pass(0, min(dt, 1e-3)) == 0
+    for c, (_, _, clamped) in zip(cells, CLAMP_PROBES):
+        if clamped:
+            assert s.W[c, 2] == F4(0.01)
+        else:
+            assert s.W[c, 2] < -0.4
+    assert s.W[1, 2] > 0.0
